@@ -1,0 +1,395 @@
+"""ctypes binding of the library's C API (include/GraphBLAS.h, include/gm_kernels.h).
+
+This is how a Python user reaches the drop-in ``GrB_mxm``: the same calls a
+C program makes (GrB_Matrix_new, GxB_Matrix_pack_CSR, GrB_mxm,
+GrB_Matrix_reduce_INT64, GxB_Matrix_unpack_CSR).  The shared library is the
+in-tree build ``graphblas_amd/lib/libgraphblas_mi355x.so``; importing this
+module without it raises, and every compute call needs a HIP device (there is
+no CPU fallback anywhere in the product).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from .workloads import Csr
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgraphblas_mi355x.so")
+
+GrB_SUCCESS = 0
+GrB_NO_VALUE = 1
+GrB_NULL_POINTER = -2
+GrB_INVALID_VALUE = -3
+GrB_DOMAIN_MISMATCH = -5
+GrB_DIMENSION_MISMATCH = -6
+GrB_OUTPUT_NOT_EMPTY = -7
+GrB_NOT_IMPLEMENTED = -8
+GrB_PANIC = -101
+GrB_OUT_OF_MEMORY = -102
+GrB_INVALID_OBJECT = -104
+
+# gm_type codes (include/gm_kernels.h)
+GM_TYPES = {"bool": 1, "int8": 2, "uint8": 3, "int16": 4, "uint16": 5, "int32": 6,
+            "uint32": 7, "int64": 8, "uint64": 9, "fp32": 10, "fp64": 11}
+NP_TYPES = {"bool": np.bool_, "int32": np.int32, "int64": np.int64, "fp64": np.float64,
+            "uint8": np.uint8, "int8": np.int8, "fp32": np.float32, "uint32": np.uint32,
+            "uint64": np.uint64, "int16": np.int16, "uint16": np.uint16}
+GRB_TYPE_NAMES = {"bool": "GrB_BOOL", "int8": "GrB_INT8", "uint8": "GrB_UINT8",
+                  "int16": "GrB_INT16", "uint16": "GrB_UINT16", "int32": "GrB_INT32",
+                  "uint32": "GrB_UINT32", "int64": "GrB_INT64", "uint64": "GrB_UINT64",
+                  "fp32": "GrB_FP32", "fp64": "GrB_FP64"}
+
+# semiring handle name -> (result type, A/B value type or None for pattern-only)
+SEMIRINGS = {
+    "GxB_PLUS_PAIR_INT64": ("int64", None),
+    "GxB_PLUS_PAIR_INT32": ("int32", None),
+    "GxB_ANY_PAIR_BOOL": ("bool", None),
+    "GxB_LOR_PAIR_BOOL": ("bool", None),
+    "GrB_PLUS_TIMES_SEMIRING_FP64": ("fp64", "fp64"),
+    "GrB_PLUS_TIMES_SEMIRING_INT64": ("int64", "int64"),
+    "GrB_MIN_PLUS_SEMIRING_INT32": ("int32", "int32"),
+    "GrB_MIN_PLUS_SEMIRING_INT64": ("int64", "int64"),
+    "GrB_MIN_PLUS_SEMIRING_FP64": ("fp64", "fp64"),
+    "GrB_MAX_PLUS_SEMIRING_INT32": ("int32", "int32"),
+    "GxB_ANY_SECONDI_INT64": ("int64", None),
+    "GxB_ANY_FIRSTJ_INT64": ("int64", None),
+}
+
+DESCRIPTORS = ["T1", "T0", "T0T1", "C", "CT1", "CT0", "CT0T1", "S", "ST1", "ST0", "ST0T1",
+               "SC", "SCT1", "SCT0", "SCT0T1", "R", "RT1", "RT0", "RT0T1", "RC", "RCT1",
+               "RCT0", "RCT0T1", "RS", "RST1", "RST0", "RST0T1", "RSC", "RSCT1", "RSCT0",
+               "RSCT0T1"]
+
+
+class GrBError(RuntimeError):
+    def __init__(self, info: int, msg: str):
+        super().__init__(f"GrB error {info}: {msg}")
+        self.info = info
+
+
+class GmCsr(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64),
+                ("p", ctypes.c_void_p), ("j", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("iso", ctypes.c_int32), ("type", ctypes.c_int32)]
+
+
+class GmCsrOut(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64), ("nvals", ctypes.c_int64),
+                ("p", ctypes.c_void_p), ("j", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("iso", ctypes.c_int32), ("type", ctypes.c_int32)]
+
+
+class GmSemiring(ctypes.Structure):
+    _fields_ = [("add_op", ctypes.c_int32), ("mult_op", ctypes.c_int32),
+                ("ztype", ctypes.c_int32)]
+
+
+class GmDot3Params(ctypes.Structure):
+    _fields_ = [("thread_max", ctypes.c_int64), ("team16_max", ctypes.c_int64),
+                ("wave_max", ctypes.c_int64)]
+
+
+class GmSaxpy3Params(ctypes.Structure):
+    _fields_ = [("wave_small_max", ctypes.c_int64), ("wave_large_max", ctypes.c_int64),
+                ("block_max", ctypes.c_int64)]
+
+
+class GmProfRecord(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64),
+                ("total_ms", ctypes.c_double)]
+
+
+class GmDeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 64), ("arch", ctypes.c_char * 32),
+                ("cu_count", ctypes.c_int32), ("lds_per_block", ctypes.c_int32),
+                ("hbm_bytes", ctypes.c_int64), ("hbm_peak_gbs", ctypes.c_double)]
+
+
+_lib = None
+_libc = ctypes.CDLL("libc.so.6")
+_libc.malloc.restype = ctypes.c_void_p
+_libc.malloc.argtypes = [ctypes.c_size_t]
+_libc.free.restype = None
+_libc.free.argtypes = [ctypes.c_void_p]
+
+
+def lib():
+    """The loaded C library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(there is no fallback path)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    U64 = ctypes.c_uint64
+    I32, I64 = ctypes.c_int32, ctypes.c_int64
+    sig = {
+        "GrB_init": ([ctypes.c_int], ctypes.c_int),
+        "GrB_finalize": ([], ctypes.c_int),
+        "GrB_Matrix_new": ([PP, P, U64, U64], ctypes.c_int),
+        "GrB_Matrix_free": ([PP], ctypes.c_int),
+        "GrB_Matrix_dup": ([PP, P], ctypes.c_int),
+        "GrB_Matrix_clear": ([P], ctypes.c_int),
+        "GrB_Matrix_nvals": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_nrows": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_ncols": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GxB_Matrix_iso": ([ctypes.POINTER(ctypes.c_bool), P], ctypes.c_int),
+        "GxB_Matrix_pack_CSR": ([P, PP, PP, PP, U64, U64, U64, ctypes.c_bool, ctypes.c_bool, P],
+                                ctypes.c_int),
+        "GxB_Matrix_unpack_CSR": ([P, PP, PP, PP, ctypes.POINTER(U64), ctypes.POINTER(U64),
+                                   ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_bool),
+                                   ctypes.POINTER(ctypes.c_bool), P], ctypes.c_int),
+        "GrB_mxm": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "GrB_Matrix_reduce_INT64": ([ctypes.POINTER(I64), P, P, P, P], ctypes.c_int),
+        "GrB_Matrix_reduce_INT32": ([ctypes.POINTER(I32), P, P, P, P], ctypes.c_int),
+        "GrB_Matrix_reduce_FP64": ([ctypes.POINTER(ctypes.c_double), P, P, P, P], ctypes.c_int),
+        "GrB_Matrix_error": ([ctypes.POINTER(ctypes.c_char_p), P], ctypes.c_int),
+        "GrB_Descriptor_new": ([PP], ctypes.c_int),
+        "GrB_Descriptor_set": ([P, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "GrB_Descriptor_free": ([PP], ctypes.c_int),
+        "GrB_Semiring_new": ([PP, P, P], ctypes.c_int),
+        "GrB_Semiring_free": ([PP], ctypes.c_int),
+        "GxB_Global_Option_set_INT32": ([ctypes.c_int, I32], ctypes.c_int),
+        "GrB_Matrix_build_INT64": ([P, P, P, P, U64, P], ctypes.c_int),
+        "GrB_Matrix_build_FP64": ([P, P, P, P, U64, P], ctypes.c_int),
+        "GrB_Matrix_build_INT32": ([P, P, P, P, U64, P], ctypes.c_int),
+        "GrB_Matrix_build_BOOL": ([P, P, P, P, U64, P], ctypes.c_int),
+        "GrB_Matrix_extractTuples_INT64": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_extractTuples_FP64": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GB_mxm_select": ([ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool,
+                           ctypes.c_int], ctypes.c_int),
+        "GB_semiring_codes": ([P, ctypes.POINTER(GmSemiring)], ctypes.c_bool),
+        # kernel layer
+        "gm_init": ([], ctypes.c_int),
+        "gm_finalize": ([], None),
+        "gm_last_error": ([], ctypes.c_char_p),
+        "gm_device_count": ([], ctypes.c_int),
+        "gm_get_device_info": ([ctypes.POINTER(GmDeviceInfo)], ctypes.c_int),
+        "gm_malloc": ([ctypes.c_size_t], P),
+        "gm_free": ([P], None),
+        "gm_arena_trim": ([], ctypes.c_int),
+        "gm_arena_bytes_in_use": ([], I64),
+        "gm_debug_fail_after": ([I64], None),
+        "gm_memcpy_h2d": ([P, P, ctypes.c_size_t], ctypes.c_int),
+        "gm_memcpy_d2h": ([P, P, ctypes.c_size_t], ctypes.c_int),
+        "gm_memcpy_d2d": ([P, P, ctypes.c_size_t], ctypes.c_int),
+        "gm_memset": ([P, ctypes.c_int, ctypes.c_size_t], ctypes.c_int),
+        "gm_synchronize": ([], ctypes.c_int),
+        "gm_prof_enable": ([ctypes.c_int], None),
+        "gm_prof_reset": ([], None),
+        "gm_prof_get": ([ctypes.POINTER(GmProfRecord), ctypes.c_int], ctypes.c_int),
+        "gm_dot3_branch": ([ctypes.POINTER(GmCsr), ctypes.c_bool, ctypes.POINTER(GmCsr),
+                            ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring)], ctypes.c_bool),
+        "gm_dot3": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.c_bool,
+                     ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
+                     ctypes.POINTER(GmDot3Params)], ctypes.c_int),
+        "gm_saxpy3_branch": ([ctypes.POINTER(GmCsr), ctypes.c_bool, ctypes.c_bool,
+                              ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
+                              ctypes.POINTER(GmSemiring)], ctypes.c_bool),
+        "gm_saxpy3": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.c_bool,
+                       ctypes.c_bool, ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
+                       ctypes.POINTER(GmSemiring), ctypes.POINTER(GmSaxpy3Params)], ctypes.c_int),
+        "gm_reduce_scalar": ([P, I32, I32, P, I64, I32], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def handle(name: str) -> ctypes.c_void_p:
+    """Value of a global handle such as GxB_PLUS_PAIR_INT64 or GrB_DESC_ST1."""
+    return ctypes.c_void_p.in_dll(lib(), name).value
+
+
+def desc(name: Optional[str]):
+    if name is None:
+        return None
+    return handle("GrB_DESC_" + name)
+
+
+_initialized = False
+
+
+def init():
+    global _initialized
+    if not _initialized:
+        rc = lib().GrB_init(0)
+        if rc not in (GrB_SUCCESS, GrB_INVALID_VALUE):
+            raise GrBError(rc, "GrB_init")
+        _initialized = True
+
+
+def _check(rc: int, what: str, obj=None):
+    if rc != GrB_SUCCESS:
+        msg = what
+        if obj is not None:
+            s = ctypes.c_char_p()
+            lib().GrB_Matrix_error(ctypes.byref(s), obj)
+            if s.value:
+                msg += ": " + s.value.decode(errors="replace")
+        raise GrBError(rc, msg)
+
+
+def _malloc_copy(a: np.ndarray) -> int:
+    a = np.ascontiguousarray(a)
+    nb = max(a.nbytes, 8)
+    p = _libc.malloc(nb)
+    if not p:
+        raise MemoryError("malloc")
+    ctypes.memmove(p, a.ctypes.data, a.nbytes)
+    return p
+
+
+class Matrix:
+    """A GrB_Matrix handle (CSR in HBM)."""
+
+    def __init__(self, type_name: str, nrows: int, ncols: int):
+        init()
+        self.type_name = type_name
+        self.h = ctypes.c_void_p()
+        _check(lib().GrB_Matrix_new(ctypes.byref(self.h), handle(GRB_TYPE_NAMES[type_name]),
+                                    nrows, ncols), "GrB_Matrix_new")
+        self.nrows, self.ncols = nrows, ncols
+
+    def __del__(self):
+        try:
+            if self.h and self.h.value:
+                lib().GrB_Matrix_free(ctypes.byref(self.h))
+        except Exception:
+            pass
+
+    @property
+    def nvals(self) -> int:
+        v = ctypes.c_uint64()
+        _check(lib().GrB_Matrix_nvals(ctypes.byref(v), self.h), "nvals", self.h)
+        return v.value
+
+    @property
+    def iso(self) -> bool:
+        v = ctypes.c_bool()
+        _check(lib().GxB_Matrix_iso(ctypes.byref(v), self.h), "iso", self.h)
+        return v.value
+
+    @classmethod
+    def from_csr(cls, m: Csr, type_name: str, iso_value=None) -> "Matrix":
+        """Pack host CSR arrays (copied) into a new matrix.  Pattern-only
+        matrices (m.x is None) become iso with value 1 / true."""
+        M = cls(type_name, m.nrows, m.ncols)
+        dt = NP_TYPES[type_name]
+        if m.x is None:
+            x = np.array([1 if iso_value is None else iso_value], dtype=dt)
+            iso = True
+        elif m.iso:
+            x = np.asarray(m.x[:1], dtype=dt)
+            iso = True
+        else:
+            x = np.asarray(m.x, dtype=dt)
+            iso = False
+        pp = ctypes.c_void_p(_malloc_copy(np.asarray(m.p, dtype=np.int64)))
+        jp = ctypes.c_void_p(_malloc_copy(np.asarray(m.j, dtype=np.int64)))
+        xp = ctypes.c_void_p(_malloc_copy(x))
+        rc = lib().GxB_Matrix_pack_CSR(M.h, ctypes.byref(pp), ctypes.byref(jp), ctypes.byref(xp),
+                                       8 * (m.nrows + 1), max(8 * m.nnz, 8), max(x.nbytes, 1),
+                                       iso, False, None)
+        if rc != GrB_SUCCESS:
+            for q in (pp, jp, xp):
+                if q.value:
+                    _libc.free(q)
+        _check(rc, "GxB_Matrix_pack_CSR", M.h)
+        return M
+
+    def to_csr(self) -> Csr:
+        """Copy to host CSR (the matrix is unchanged)."""
+        D = ctypes.c_void_p()
+        _check(lib().GrB_Matrix_dup(ctypes.byref(D), self.h), "GrB_Matrix_dup", self.h)
+        pp, jp, xp = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        ps, js, xs = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        iso, jum = ctypes.c_bool(), ctypes.c_bool()
+        rc = lib().GxB_Matrix_unpack_CSR(D, ctypes.byref(pp), ctypes.byref(jp), ctypes.byref(xp),
+                                         ctypes.byref(ps), ctypes.byref(js), ctypes.byref(xs),
+                                         ctypes.byref(iso), ctypes.byref(jum), None)
+        try:
+            _check(rc, "GxB_Matrix_unpack_CSR", D)
+            n = self.nrows
+            p = np.ctypeslib.as_array((ctypes.c_int64 * (n + 1)).from_address(pp.value)).copy()
+            nz = int(p[n])
+            j = np.ctypeslib.as_array((ctypes.c_int64 * max(nz, 1)).from_address(jp.value))[:nz].copy()
+            dt = NP_TYPES[self.type_name]
+            nx = 1 if iso.value else max(nz, 1)
+            xb = (ctypes.c_uint8 * (nx * np.dtype(dt).itemsize)).from_address(xp.value)
+            x = np.frombuffer(bytes(xb), dtype=dt)
+            x = np.full(nz, x[0], dtype=dt) if iso.value else x[:nz].copy()
+            return Csr(n, self.ncols, p, j, x, False)
+        finally:
+            for q in (pp, jp, xp):
+                if q.value:
+                    _libc.free(q)
+            lib().GrB_Matrix_free(ctypes.byref(D))
+
+
+def mxm(C: Matrix, M: Optional[Matrix], semiring: str, A: Matrix, B: Matrix,
+        descriptor: Optional[str] = None, accum=None) -> int:
+    """GrB_mxm(C, M, accum, semiring, A, B, desc); returns the GrB_Info code."""
+    return lib().GrB_mxm(C.h, M.h if M is not None else None, accum, handle(semiring),
+                         A.h, B.h, desc(descriptor))
+
+
+def mxm_checked(C, M, semiring, A, B, descriptor=None):
+    _check(mxm(C, M, semiring, A, B, descriptor), "GrB_mxm", C.h)
+
+
+def reduce_sum_int64(A: Matrix) -> int:
+    v = ctypes.c_int64(0)
+    _check(lib().GrB_Matrix_reduce_INT64(ctypes.byref(v), None, handle("GrB_PLUS_MONOID_INT64"),
+                                         A.h, None), "reduce", A.h)
+    return v.value
+
+
+def reduce_sum_fp64(A: Matrix) -> float:
+    v = ctypes.c_double(0)
+    _check(lib().GrB_Matrix_reduce_FP64(ctypes.byref(v), None, handle("GrB_PLUS_MONOID_FP64"),
+                                        A.h, None), "reduce", A.h)
+    return v.value
+
+
+def set_burble(on: bool):
+    lib().GxB_Global_Option_set_INT32(7019, 1 if on else 0)
+
+
+def last_error() -> str:
+    s = lib().gm_last_error()
+    return s.decode(errors="replace") if s else ""
+
+
+def device_info() -> dict:
+    info = GmDeviceInfo()
+    rc = lib().gm_get_device_info(ctypes.byref(info))
+    if rc != 0:
+        raise GrBError(rc, last_error())
+    return {"name": info.name.decode(), "arch": info.arch.decode(), "cu_count": info.cu_count,
+            "lds_per_block": info.lds_per_block, "hbm_bytes": info.hbm_bytes,
+            "hbm_peak_gbs": info.hbm_peak_gbs}
+
+
+def prof_enable(on=True):
+    lib().gm_prof_enable(1 if on else 0)
+
+
+def prof_reset():
+    lib().gm_prof_reset()
+
+
+def prof_get() -> dict:
+    recs = (GmProfRecord * 64)()
+    n = lib().gm_prof_get(recs, 64)
+    return {recs[t].name.decode(): (recs[t].launches, recs[t].total_ms) for t in range(min(n, 64))}

@@ -1,0 +1,620 @@
+// gm_dot3.hip: masked dot-product SpGEMM, C<M> = A*B' (the reference's dot3).
+//
+// Reference: Source/mxm/GB_AxB_dot3.c:37-357 (driver), template/
+// GB_AxB_dot3_template.c:13-169 (per mask entry, zombies) and
+// template/GB_AxB_dot_cij.c:516-652 (sparse x sparse dot product).
+//
+// All operands are CSR.  For each mask entry (i,j): C(i,j) = A(i,:) . B(j,:)
+// under the semiring.  Work is organised per mask entry (never per row, so a
+// hub row of M does not serialise):
+//
+//  phase1  one thread per entry: row lookup, mask test, sizes a=|A(i,:)|,
+//          b=|B(j,:)|, work = min(a+b, s*ceil(log2(l+1))).  Small work is
+//          done in place by that thread; larger entries are appended
+//          (wavefront-aggregated atomics) to one of three bins.
+//  team16 / wave / block: 16, 64 or 256 lanes per entry.  A balanced pair is
+//          split by merge-path diagonals; an unbalanced pair (s*log l < a+b)
+//          is split over the short list, each lane binary-searching the long
+//          one.  Lanes reduce in lane order (= k order), so fp64 sums are
+//          deterministic and ANY keeps the first match in k order, like the
+//          reference's sequential merge.
+//  compact one pass per 2048-entry tile: block scan of the live flags, scatter
+//          of the surviving (j, value) pairs, and the new row pointers of all
+//          rows that start in the tile.  The result has no zombies.
+
+#include "gm_internal.h"
+#include "gm_block.h"
+
+namespace gm
+{
+
+constexpr int D3_NT = 256;            // threads per workgroup everywhere here
+constexpr int D3_P1_PER_THREAD = 4;   // entries per thread per phase1 tile
+constexpr int D3_P1_TILE = D3_NT * D3_P1_PER_THREAD;
+constexpr int D3_CT_IPT = 8;          // compaction items per thread
+constexpr int D3_CT_TILE = D3_NT * D3_CT_IPT;
+
+struct Dot3Args
+{
+    const int64_t *Mp, *Mj;
+    const void *Mx;
+    int32_t msize, m_iso, mask_struct;
+    int64_t m_nrows, mnz;
+    const int64_t *Ap, *Aj;
+    const void *Ax;
+    int32_t a_iso;
+    const int64_t *Bp, *Bj;
+    const void *Bx;
+    int32_t b_iso;
+    void *W;                      // one Z per mask entry
+    uint8_t *live;                // 1 if C(i,j) exists
+    uint64_t *bins;               // 3 bins x mnz, packed (i << 32) | p
+    unsigned long long *bin_cnt;  // 3 counters
+    int64_t thread_max, team16_max, wave_max;
+};
+
+// lane-aggregated append: returns the slot for lanes with pred set.
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned long long *counter)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long) mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long) __popcll(mask));
+    base = __shfl(base, leader, 64);
+    const unsigned long long below = mask & ((1ull << lane) - 1ull);
+    return base + (unsigned long long) __popcll(below);
+}
+
+__device__ __forceinline__ int64_t d3_work(int64_t a, int64_t b)
+{
+    const int64_t s = a < b ? a : b, l = a < b ? b : a;
+    const int64_t srch = s * (int64_t) (ilog2_ceil(l + 1) + 1);
+    return srch < a + b ? srch : a + b;
+}
+
+// Single-thread dot product of A(i,:) and B(j,:): a linear merge, or a
+// binary trim of the longer list when it is much longer (the reference's
+// rule at GB_AxB_dot_cij.c:538-616, here chosen by estimated cost).
+template <class S>
+__device__ bool dot_serial(const int64_t *__restrict__ Aj, const typename S::Z *__restrict__ Ax,
+                           int a_iso, int64_t pa, int64_t pa_end,
+                           const int64_t *__restrict__ Bj, const typename S::Z *__restrict__ Bx,
+                           int b_iso, int64_t pb, int64_t pb_end, typename S::Z &out)
+{
+    using Z = typename S::Z;
+    const int64_t a = pa_end - pa, b = pb_end - pb;
+    if (a == 0 || b == 0) return false;
+    if (Aj[pa_end - 1] < Bj[pb] || Bj[pb_end - 1] < Aj[pa]) return false;
+    bool e = false;
+    Z z = S::identity();
+    const bool trimA = a > 8 * b, trimB = b > 8 * a;
+    int64_t ia = Aj[pa], ib = Bj[pb];
+    while (true)
+    {
+        if (ia < ib)
+        {
+            if (trimA) pa = lower_bound(Aj, pa + 1, pa_end, ib); else pa++;
+            if (pa >= pa_end) break;
+            ia = Aj[pa];
+        }
+        else if (ib < ia)
+        {
+            if (trimB) pb = lower_bound(Bj, pb + 1, pb_end, ia); else pb++;
+            if (pb >= pb_end) break;
+            ib = Bj[pb];
+        }
+        else
+        {
+            Z t = S::mul(S::geta(Ax, pa, a_iso), S::geta(Bx, pb, b_iso), ia);
+            if (!e) { z = t; e = true; } else z = S::upd(z, t);
+            if constexpr (S::is_any) break;
+            if (S::terminal(z)) break;
+            pa++;
+            pb++;
+            if (pa >= pa_end || pb >= pb_end) break;
+            ia = Aj[pa];
+            ib = Bj[pb];
+        }
+    }
+    if constexpr (S::plus_pair_wide) e = (z != 0);
+    out = z;
+    return e;
+}
+
+// One lane's share of a team dot product.  lane in [0,G).  Returns the
+// lane's partial (e, z); partials combine in lane order.
+template <class S, int G>
+__device__ void dot_lane(int lane, const int64_t *__restrict__ Aj, const typename S::Z *__restrict__ Ax,
+                         int a_iso, int64_t pa, int64_t pa_end,
+                         const int64_t *__restrict__ Bj, const typename S::Z *__restrict__ Bx,
+                         int b_iso, int64_t pb, int64_t pb_end, bool &e, typename S::Z &z)
+{
+    using Z = typename S::Z;
+    e = false;
+    z = S::identity();
+    const int64_t a = pa_end - pa, b = pb_end - pb;
+    if (a == 0 || b == 0) return;
+    if (Aj[pa_end - 1] < Bj[pb] || Bj[pb_end - 1] < Aj[pa]) return;
+    const int64_t s = a < b ? a : b, l = a < b ? b : a;
+    const bool search = s * (int64_t) (ilog2_ceil(l + 1) + 1) < a + b;
+    if (search)
+    {
+        // split the short list into G contiguous chunks; binary search each
+        // element in the long list from the previous hit onward
+        const bool shortA = a <= b;
+        const int64_t *Sj = shortA ? Aj : Bj;
+        const int64_t *Lj = shortA ? Bj : Aj;
+        const int64_t s0 = shortA ? pa : pb;
+        int64_t lcur = shortA ? pb : pa;
+        const int64_t lend = shortA ? pb_end : pa_end;
+        const int64_t chunk = (s + G - 1) / G;
+        const int64_t q0 = s0 + lane * chunk;
+        const int64_t q1 = (s0 + s) < (q0 + chunk) ? (s0 + s) : (q0 + chunk);
+        for (int64_t q = q0; q < q1; q++)
+        {
+            const int64_t x = Sj[q];
+            lcur = lower_bound(Lj, lcur, lend, x);
+            if (lcur >= lend) break;
+            if (Lj[lcur] == x)
+            {
+                const int64_t qa = shortA ? q : lcur, qb = shortA ? lcur : q;
+                Z t = S::mul(S::geta(Ax, qa, a_iso), S::geta(Bx, qb, b_iso), x);
+                if (!e) { z = t; e = true; } else z = S::upd(z, t);
+                if constexpr (S::is_any) break;
+                if (S::terminal(z)) break;
+                lcur++;
+            }
+        }
+    }
+    else
+    {
+        // merge path: diagonals d0..d1 of the (a+b)-long merge, A first on
+        // ties; a match is taken when B[y] is consumed right after an equal
+        // A[x-1], so each common index is seen by exactly one lane
+        const int64_t tot = a + b;
+        const int64_t d0 = (tot * lane) / G, d1 = (tot * (lane + 1)) / G;
+        int64_t lo = d0 > b ? d0 - b : 0, hi = d0 < a ? d0 : a;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi) >> 1;
+            if (Aj[pa + mid] <= Bj[pb + d0 - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        int64_t x = lo, y = d0 - lo;
+        for (int64_t d = d0; d < d1; d++)
+        {
+            if (x < a && (y >= b || Aj[pa + x] <= Bj[pb + y]))
+            {
+                x++;
+            }
+            else
+            {
+                const int64_t kb = Bj[pb + y];
+                if (x > 0 && Aj[pa + x - 1] == kb)
+                {
+                    Z t = S::mul(S::geta(Ax, pa + x - 1, a_iso), S::geta(Bx, pb + y, b_iso), kb);
+                    if (!e) { z = t; e = true; } else z = S::upd(z, t);
+                    if constexpr (S::is_any) break;
+                    if (S::terminal(z)) break;
+                }
+                y++;
+            }
+        }
+    }
+}
+
+// final existence rule and store of one entry
+template <class S>
+__device__ __forceinline__ void d3_store(const Dot3Args &g, int64_t p, bool e, typename S::Z z)
+{
+    if constexpr (S::plus_pair_wide) e = (z != 0);
+    g.live[p] = e ? 1 : 0;
+    if (e && !S::iso_out) ((typename S::Z *) g.W)[p] = z;
+}
+
+//------------------------------------------------------------------------------
+// phase1: classify every mask entry, finish the small ones
+//------------------------------------------------------------------------------
+
+template <class S>
+__global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
+{
+    using Z = typename S::Z;
+    __shared__ int64_t s_rows[2];
+    const int64_t ntiles = (g.mnz + D3_P1_TILE - 1) / D3_P1_TILE;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    {
+        const int64_t t0 = tile * D3_P1_TILE;
+        const int64_t t1 = (t0 + D3_P1_TILE < g.mnz) ? t0 + D3_P1_TILE : g.mnz;
+        if (threadIdx.x < 2)
+        {
+            // row containing entry t0 (thread 0) / t1-1 (thread 1)
+            const int64_t pp = threadIdx.x == 0 ? t0 : t1 - 1;
+            int64_t lo = 0, hi = g.m_nrows - 1;
+            while (lo < hi)
+            {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (g.Mp[mid] <= pp) lo = mid; else hi = mid - 1;
+            }
+            s_rows[threadIdx.x] = lo;
+        }
+        __syncthreads();
+        const int64_t r0 = s_rows[0], r1 = s_rows[1];
+        #pragma unroll 1
+        for (int u = 0; u < D3_P1_PER_THREAD; u++)
+        {
+            const int64_t p = t0 + (int64_t) u * D3_NT + threadIdx.x;
+            const bool valid = p < t1;
+            int bin = -1;
+            int64_t i = 0;
+            if (valid)
+            {
+                int64_t lo = r0, hi = r1;
+                while (lo < hi)
+                {
+                    const int64_t mid = (lo + hi + 1) >> 1;
+                    if (g.Mp[mid] <= p) lo = mid; else hi = mid - 1;
+                }
+                i = lo;
+                const int64_t j = g.Mj[p];
+                bool e = false;
+                Z z = S::identity();
+                if (g.mask_struct || mask_true(g.Mx, p, g.msize, g.m_iso))
+                {
+                    const int64_t pa = g.Ap[i], pa_end = g.Ap[i + 1];
+                    const int64_t pb = g.Bp[j], pb_end = g.Bp[j + 1];
+                    const int64_t a = pa_end - pa, b = pb_end - pb;
+                    if (a > 0 && b > 0)
+                    {
+                        const int64_t w = d3_work(a, b);
+                        if (w <= g.thread_max)
+                        {
+                            e = dot_serial<S>(g.Aj, (const Z *) g.Ax, g.a_iso, pa, pa_end,
+                                              g.Bj, (const Z *) g.Bx, g.b_iso, pb, pb_end, z);
+                        }
+                        else
+                        {
+                            bin = (w <= g.team16_max) ? 0 : (w <= g.wave_max) ? 1 : 2;
+                        }
+                    }
+                }
+                if (bin < 0)
+                {
+                    g.live[p] = e ? 1 : 0;
+                    if (e && !S::iso_out) ((Z *) g.W)[p] = z;
+                }
+            }
+            #pragma unroll
+            for (int b = 0; b < 3; b++)
+            {
+                const bool take = (bin == b);
+                const unsigned long long slot = wave_append(take, &g.bin_cnt[b]);
+                if (take) g.bins[(int64_t) b * g.mnz + (int64_t) slot] = ((uint64_t) i << 32) | (uint64_t) p;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+//------------------------------------------------------------------------------
+// team kernels: G lanes per deferred entry (G = 16 or 64)
+//------------------------------------------------------------------------------
+
+template <class S, int G>
+__global__ __launch_bounds__(D3_NT) void k_dot3_team(Dot3Args g, int bin)
+{
+    using Z = typename S::Z;
+    const int64_t count = (int64_t) g.bin_cnt[bin];
+    const uint64_t *list = g.bins + (int64_t) bin * g.mnz;
+    constexpr int TPB = D3_NT / G;
+    const int lane = threadIdx.x & (G - 1);
+    for (int64_t q = (int64_t) blockIdx.x * TPB + threadIdx.x / G; q < count;
+         q += (int64_t) gridDim.x * TPB)
+    {
+        const uint64_t w = list[q];
+        const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+        const int64_t j = g.Mj[p];
+        bool e;
+        Z z;
+        dot_lane<S, G>(lane, g.Aj, (const Z *) g.Ax, g.a_iso, g.Ap[i], g.Ap[i + 1],
+                       g.Bj, (const Z *) g.Bx, g.b_iso, g.Bp[j], g.Bp[j + 1], e, z);
+        team_reduce<S, G>(e, z);
+        if (lane == 0) d3_store<S>(g, p, e, z);
+    }
+}
+
+// one 256-thread workgroup per entry (the heaviest bin)
+template <class S>
+__global__ __launch_bounds__(D3_NT) void k_dot3_block(Dot3Args g)
+{
+    using Z = typename S::Z;
+    __shared__ Z s_z[D3_NT / 64];
+    __shared__ int s_e[D3_NT / 64];
+    const int64_t count = (int64_t) g.bin_cnt[2];
+    const uint64_t *list = g.bins + 2 * g.mnz;
+    const int wid = threadIdx.x >> 6;
+    for (int64_t q = blockIdx.x; q < count; q += gridDim.x)
+    {
+        const uint64_t w = list[q];
+        const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+        const int64_t j = g.Mj[p];
+        bool e;
+        Z z;
+        dot_lane<S, D3_NT>(threadIdx.x, g.Aj, (const Z *) g.Ax, g.a_iso, g.Ap[i], g.Ap[i + 1],
+                           g.Bj, (const Z *) g.Bx, g.b_iso, g.Bp[j], g.Bp[j + 1], e, z);
+        team_reduce<S, 64>(e, z);
+        if ((threadIdx.x & 63) == 0) { s_z[wid] = z; s_e[wid] = e ? 1 : 0; }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            bool ee = s_e[0] != 0;
+            Z zz = s_z[0];
+            for (int t = 1; t < D3_NT / 64; t++) combine<S>(ee, zz, s_e[t] != 0, s_z[t]);
+            d3_store<S>(g, p, ee, zz);
+        }
+        __syncthreads();
+    }
+}
+
+//------------------------------------------------------------------------------
+// compaction: drop zombies, build the new row pointers
+//------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(D3_NT) void k_dot3_tile_count(const uint8_t *__restrict__ live,
+                                                           int64_t mnz, int64_t *__restrict__ cnt)
+{
+    __shared__ int64_t lds[D3_NT / 64 + 1];
+    const int64_t base = (int64_t) blockIdx.x * D3_CT_TILE + (int64_t) threadIdx.x * D3_CT_IPT;
+    int64_t c = 0;
+    #pragma unroll
+    for (int t = 0; t < D3_CT_IPT; t++)
+    {
+        const int64_t p = base + t;
+        if (p < mnz) c += live[p];
+    }
+    const int64_t tot = block_sum<int64_t, D3_NT>(c, lds);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+template <class Z, bool ISO>
+__global__ __launch_bounds__(D3_NT) void k_dot3_compact(const int64_t *__restrict__ Mp,
+    const int64_t *__restrict__ Mj, int64_t m_nrows, int64_t mnz,
+    const uint8_t *__restrict__ live, const Z *__restrict__ W,
+    const int64_t *__restrict__ tile_off, int64_t *__restrict__ Cp,
+    int64_t *__restrict__ Ci, Z *__restrict__ Cx)
+{
+    __shared__ int64_t lds[D3_NT / 64 + 1];
+    __shared__ int32_t s_pre[D3_CT_TILE + 1];
+    __shared__ int64_t s_row0;
+    const int64_t t0 = (int64_t) blockIdx.x * D3_CT_TILE;
+    const int64_t base = t0 + (int64_t) threadIdx.x * D3_CT_IPT;
+    uint8_t f[D3_CT_IPT];
+    int64_t c = 0;
+    #pragma unroll
+    for (int t = 0; t < D3_CT_IPT; t++)
+    {
+        const int64_t p = base + t;
+        f[t] = (p < mnz) ? live[p] : 0;
+        c += f[t];
+    }
+    int64_t tot;
+    const int64_t ex = block_exclusive_scan<int64_t, D3_NT>(c, lds, &tot);
+    const int64_t off = tile_off[blockIdx.x];
+    int64_t run = ex;
+    #pragma unroll
+    for (int t = 0; t < D3_CT_IPT; t++)
+    {
+        const int64_t p = base + t;
+        s_pre[threadIdx.x * D3_CT_IPT + t] = (int32_t) run;
+        if (f[t])
+        {
+            Ci[off + run] = Mj[p];
+            if constexpr (!ISO) Cx[off + run] = W[p];
+        }
+        run += f[t];
+    }
+    if (threadIdx.x == D3_NT - 1) s_pre[D3_CT_TILE] = (int32_t) run;
+    if (threadIdx.x == 0)
+    {
+        // first row whose start lies at or after t0
+        int64_t lo = 0, hi = m_nrows + 1;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi) >> 1;
+            if (Mp[mid] < t0) lo = mid + 1; else hi = mid;
+        }
+        s_row0 = lo;
+    }
+    __syncthreads();
+    // rows starting inside this tile; the last tile also takes rows that
+    // start at mnz (trailing empty rows and the final pointer)
+    const bool last = (t0 + D3_CT_TILE >= mnz);
+    const int64_t t_end = last ? mnz + 1 : t0 + D3_CT_TILE;
+    for (int64_t r = s_row0 + threadIdx.x; r <= m_nrows; r += D3_NT)
+    {
+        const int64_t mp = Mp[r];
+        if (mp >= t_end) break;
+        Cp[r] = off + s_pre[mp - t0];
+    }
+}
+
+__global__ void k_fill_i64(int64_t *x, int64_t n, int64_t v)
+{
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t) gridDim.x * blockDim.x)
+        x[t] = v;
+}
+
+__global__ void k_set_iso_one(void *x, int size)
+{
+    // iso value of ANY_PAIR: true / 1
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        uint8_t *b = (uint8_t *) x;
+        for (int t = 0; t < size; t++) b[t] = 0;
+        b[0] = 1;
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long *x, int n)
+{
+    if (threadIdx.x < n) x[threadIdx.x] = 0;
+}
+
+static inline unsigned grid_for(int64_t work_items, int64_t per_block, int64_t cap)
+{
+    int64_t g = (work_items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned) g;
+}
+
+template <class S>
+static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                    const gm_csr *B, const gm_dot3_params *prm)
+{
+    using Z = typename S::Z;
+    hipStream_t st = stream();
+    const int64_t n = M->nrows, mnz = (n > 0) ? -1 : 0;
+    (void) mnz;
+    int64_t Mnz = 0;
+    {
+        // nnz(M) from its last row pointer
+        int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
+        if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+        GM_HIP(hipMemcpyAsync(hp, M->p + n, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(M)");
+        GM_HIP(hipStreamSynchronize(st), "nnz(M) sync");
+        Mnz = *hp;
+    }
+    if (Mnz >= ((int64_t) 1 << 32) || n >= ((int64_t) 1 << 32))
+        return set_error(GM_NO_VALUE, "dot3: nnz(M) or nrows beyond 2^32");
+
+    const size_t zs = sizeof(Z);
+    DevBuf Cp, Ci, Cx, W, live, bins, cnt;
+    if (!Cp.alloc(sizeof(int64_t) * (n + 1)) || !Ci.alloc(sizeof(int64_t) * (Mnz > 0 ? Mnz : 1)) ||
+        !Cx.alloc(S::iso_out ? zs : zs * (Mnz > 0 ? Mnz : 1)))
+        return set_error(GM_OUT_OF_MEMORY, "dot3: output");
+    int64_t nvals = 0;
+    if (Mnz > 0)
+    {
+        if (!W.alloc(S::iso_out ? 8 : zs * Mnz) || !live.alloc(Mnz) ||
+            !bins.alloc(sizeof(uint64_t) * 3 * Mnz) || !cnt.alloc(4 * sizeof(unsigned long long)))
+            return set_error(GM_OUT_OF_MEMORY, "dot3: workspace");
+        Dot3Args g;
+        g.Mp = M->p; g.Mj = M->j; g.Mx = mask_struct ? nullptr : M->x;
+        g.msize = (int32_t) type_size(M->type); g.m_iso = M->iso; g.mask_struct = mask_struct ? 1 : 0;
+        if (!mask_struct && M->x == nullptr) g.mask_struct = 1;
+        g.m_nrows = n; g.mnz = Mnz;
+        g.Ap = A->p; g.Aj = A->j; g.Ax = A->x; g.a_iso = A->iso;
+        g.Bp = B->p; g.Bj = B->j; g.Bx = B->x; g.b_iso = B->iso;
+        g.W = W.ptr; g.live = live.as<uint8_t>(); g.bins = bins.as<uint64_t>();
+        g.bin_cnt = cnt.as<unsigned long long>();
+        g.thread_max = (prm && prm->thread_max > 0) ? prm->thread_max : 32;
+        g.team16_max = (prm && prm->team16_max > 0) ? prm->team16_max : 512;
+        g.wave_max = (prm && prm->wave_max > 0) ? prm->wave_max : 4096;
+        const int64_t cap = (int64_t) cu_count() * 16;
+        k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
+        {
+            ProfScope ps("dot3_phase1");
+            k_dot3_phase1<S><<<grid_for(Mnz, D3_P1_TILE, cap), D3_NT, 0, st>>>(g);
+            GM_HIP(hipGetLastError(), "k_dot3_phase1");
+        }
+        {
+            ProfScope ps("dot3_team16");
+            k_dot3_team<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 0);
+            GM_HIP(hipGetLastError(), "k_dot3_team16");
+        }
+        {
+            ProfScope ps("dot3_wave");
+            k_dot3_team<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 1);
+            GM_HIP(hipGetLastError(), "k_dot3_wave");
+        }
+        {
+            ProfScope ps("dot3_block");
+            k_dot3_block<S><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g);
+            GM_HIP(hipGetLastError(), "k_dot3_block");
+        }
+        // compaction
+        const int64_t ntiles = (Mnz + D3_CT_TILE - 1) / D3_CT_TILE;
+        DevBuf toff;
+        if (!toff.alloc(sizeof(int64_t) * (ntiles + 1)))
+            return set_error(GM_OUT_OF_MEMORY, "dot3: compaction workspace");
+        {
+            ProfScope ps("dot3_tile_count");
+            k_dot3_tile_count<<<(unsigned) ntiles, D3_NT, 0, st>>>(live.as<uint8_t>(), Mnz,
+                                                                   toff.as<int64_t>());
+            GM_HIP(hipGetLastError(), "k_dot3_tile_count");
+        }
+        GM_TRY(scan_exclusive_i64(toff.as<int64_t>(), toff.as<int64_t>(), ntiles));
+        {
+            ProfScope ps("dot3_compact");
+            k_dot3_compact<Z, S::iso_out><<<(unsigned) ntiles, D3_NT, 0, st>>>(
+                M->p, M->j, n, Mnz, live.as<uint8_t>(), (const Z *) W.ptr, toff.as<int64_t>(),
+                Cp.as<int64_t>(), Ci.as<int64_t>(), (Z *) Cx.ptr);
+            GM_HIP(hipGetLastError(), "k_dot3_compact");
+        }
+        int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
+        GM_HIP(hipMemcpyAsync(hp, toff.as<int64_t>() + ntiles, sizeof(int64_t),
+                              hipMemcpyDeviceToHost, st), "nvals");
+        GM_HIP(hipStreamSynchronize(st), "dot3 sync");
+        nvals = *hp;
+    }
+    else
+    {
+        k_fill_i64<<<grid_for(n + 1, 256, 1024), 256, 0, st>>>(Cp.as<int64_t>(), n + 1, 0);
+        GM_HIP(hipGetLastError(), "k_fill_i64");
+    }
+    if (S::iso_out)
+    {
+        k_set_iso_one<<<1, 1, 0, st>>>(Cx.ptr, (int) zs);
+        GM_HIP(hipGetLastError(), "k_set_iso_one");
+    }
+    C->nrows = n;
+    C->ncols = B->nrows;
+    C->nvals = nvals;
+    C->p = (int64_t *) Cp.release();
+    C->j = (int64_t *) Ci.release();
+    C->x = Cx.release();
+    C->iso = S::iso_out ? 1 : 0;
+    C->type = 0;
+    return GM_SUCCESS;
+}
+
+} // namespace gm
+
+using namespace gm;
+
+extern "C" bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
+                               const gm_csr *B, const gm_semiring *sr)
+{
+    (void) mask_struct;
+    if (!M || !A || !B || !sr) return false;
+    if (!semiring_supported(sr)) return false;
+    if (!operand_ok(sr, A) || !operand_ok(sr, B)) return false;
+    // C<M> = A*B': M is nrows(A) x nrows(B), A and B share their column space
+    if (M->nrows != A->nrows || M->ncols != B->nrows || A->ncols != B->ncols) return false;
+    if (M->nrows >= ((int64_t) 1 << 32)) return false;
+    return true;
+}
+
+extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                       const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
+{
+    if (!C || !M || !A || !B || !sr) return set_error(GM_INVALID_VALUE, "gm_dot3: NULL argument");
+    if (!gm_dot3_branch(M, mask_struct, A, B, sr))
+        return set_error(GM_NO_VALUE, "gm_dot3: case not handled by the GPU kernels");
+    GM_TRY(ensure_init());
+    int rc = GM_NO_VALUE;
+#define GM_DOT3_CASE(ADD, MULT, ZC, Z)                                         \
+    if (sr->add_op == ADD && sr->mult_op == MULT && sr->ztype == ZC)           \
+    {                                                                          \
+        rc = dot3_run<SR<ADD, MULT, Z>>(C, M, mask_struct, A, B, params);      \
+        if (rc == GM_SUCCESS) C->type = ZC;                                    \
+        return rc;                                                             \
+    }
+    GM_SEMIRING_LIST(GM_DOT3_CASE)
+#undef GM_DOT3_CASE
+    return rc;
+}

@@ -1,0 +1,510 @@
+// gm_runtime.hip: device selection, stream, caching HBM arena, errors,
+// per-kernel event timing and the device exclusive scan.
+//
+// One non-blocking stream at default priority carries every kernel and copy
+// of the library, so stream order alone makes reuse of freed arena blocks
+// safe.  The arena keeps freed blocks in size-keyed free lists (HBM is large;
+// hipMalloc/hipFree are slow and synchronising), and gives them back on
+// allocation failure or gm_arena_trim().
+
+#include "gm_internal.h"
+#include "gm_block.h"
+
+#include <mutex>
+#include <map>
+#include <unordered_map>
+#include <vector>
+#include <string>
+#include <cstdio>
+#include <cstdarg>
+#include <cstring>
+
+namespace gm
+{
+
+static std::mutex g_mu;
+static int g_init_rc = 1;             // 1: not yet initialised
+static int g_device = -1;
+static hipStream_t g_stream = nullptr;
+static int g_cu = 0;
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return GM_SUCCESS;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return set_error(GM_OUT_OF_MEMORY, "%s: %s", what, hipGetErrorString(e));
+    return set_error(GM_PANIC, "%s: %s", what, hipGetErrorString(e));
+}
+
+hipStream_t stream() { return g_stream; }
+int cu_count() { return g_cu; }
+
+int ensure_init()
+{
+    if (g_init_rc == GM_SUCCESS) return GM_SUCCESS;
+    return gm_init();
+}
+
+size_t type_size(int32_t type)
+{
+    switch (type)
+    {
+        case GM_BOOL: case GM_INT8: case GM_UINT8: return 1;
+        case GM_INT16: case GM_UINT16: return 2;
+        case GM_INT32: case GM_UINT32: case GM_FP32: return 4;
+        case GM_INT64: case GM_UINT64: case GM_FP64: return 8;
+        default: return 0;
+    }
+}
+
+bool semiring_supported(const gm_semiring *sr)
+{
+    if (!sr) return false;
+#define GM_SR_MATCH(ADD, MULT, ZC, Z) \
+    if (sr->add_op == ADD && sr->mult_op == MULT && sr->ztype == ZC) return true;
+    GM_SEMIRING_LIST(GM_SR_MATCH)
+#undef GM_SR_MATCH
+    return false;
+}
+
+bool operand_ok(const gm_semiring *sr, const gm_csr *X)
+{
+    if (sr->mult_op == GM_MULT_PAIR || sr->mult_op == GM_MULT_K) return true;
+    return X->x != nullptr && X->type == sr->ztype;
+}
+
+bool csr_ok(const gm_csr *X)
+{
+    return X && X->p && X->nrows >= 0 && X->ncols >= 0 && (X->j || X->nrows == 0 || true);
+}
+
+//------------------------------------------------------------------------------
+// arena
+//------------------------------------------------------------------------------
+
+static std::multimap<size_t, void *> g_free;        // size -> block
+static std::unordered_map<void *, size_t> g_live;   // block -> size
+static size_t g_cached_bytes = 0, g_live_bytes = 0;
+static int64_t g_fail_after = -1;
+
+static size_t round_size(size_t b)
+{
+    if (b < 256) return 256;
+    if (b < ((size_t) 1 << 20))
+    {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t g = (size_t) 2 << 20;
+    return (b + g - 1) / g * g;
+}
+
+static void trim_locked()
+{
+    for (auto &kv : g_free) (void) hipFree(kv.second);
+    g_free.clear();
+    g_cached_bytes = 0;
+}
+
+//------------------------------------------------------------------------------
+// profiling
+//------------------------------------------------------------------------------
+
+static bool g_prof = false;
+struct ProfEv { int name_id; hipEvent_t a, b; };
+static std::vector<ProfEv> g_prof_evs;
+static std::vector<std::string> g_prof_names;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t get_event()
+{
+    if (!g_event_pool.empty())
+    {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void) hipEventCreate(&e);
+    return e;
+}
+
+ProfScope::ProfScope(const char *name) : slot(-1)
+{
+    if (!g_prof) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int id = -1;
+    for (size_t t = 0; t < g_prof_names.size(); t++)
+        if (g_prof_names[t] == name) { id = (int) t; break; }
+    if (id < 0) { g_prof_names.emplace_back(name); id = (int) g_prof_names.size() - 1; }
+    ProfEv ev{id, get_event(), get_event()};
+    (void) hipEventRecord(ev.a, g_stream);
+    g_prof_evs.push_back(ev);
+    slot = (int) g_prof_evs.size() - 1;
+}
+
+ProfScope::~ProfScope()
+{
+    if (slot < 0) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void) hipEventRecord(g_prof_evs[slot].b, g_stream);
+}
+
+static void *g_pinned = nullptr;
+static size_t g_pinned_bytes = 0;
+
+void *pinned_scratch(size_t bytes)
+{
+    if (bytes <= g_pinned_bytes) return g_pinned;
+    if (g_pinned) (void) hipHostFree(g_pinned);
+    g_pinned = nullptr;
+    g_pinned_bytes = 0;
+    size_t want = bytes < 65536 ? 65536 : bytes;
+    if (hipHostMalloc(&g_pinned, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+    g_pinned_bytes = want;
+    return g_pinned;
+}
+
+//------------------------------------------------------------------------------
+// exclusive scan (int64): reduce tiles -> scan tile sums -> rescan tiles
+//------------------------------------------------------------------------------
+
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_IPT = 8;
+constexpr int SCAN_TILE = SCAN_NT * SCAN_IPT;
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_reduce(const int64_t *__restrict__ in,
+                                                         int64_t n, int64_t *__restrict__ sums)
+{
+    __shared__ int64_t lds[SCAN_NT / 64 + 1];
+    const int64_t base = (int64_t) blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+    #pragma unroll
+    for (int t = 0; t < SCAN_IPT; t++)
+    {
+        int64_t q = base + (int64_t) t * SCAN_NT + threadIdx.x;
+        if (q < n) s += in[q];
+    }
+    int64_t tot = block_sum<int64_t, SCAN_NT>(s, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_apply(const int64_t *in, int64_t n,
+                                                        const int64_t *__restrict__ offs,
+                                                        int64_t *out)
+{
+    __shared__ int64_t lds[SCAN_NT / 64 + 1];
+    const int64_t base = (int64_t) blockIdx.x * SCAN_TILE + (int64_t) threadIdx.x * SCAN_IPT;
+    int64_t v[SCAN_IPT];
+    int64_t s = 0;
+    #pragma unroll
+    for (int t = 0; t < SCAN_IPT; t++)
+    {
+        int64_t q = base + t;
+        v[t] = (q < n) ? in[q] : 0;
+        s += v[t];
+    }
+    int64_t tot;
+    int64_t ex = block_exclusive_scan<int64_t, SCAN_NT>(s, lds, &tot);
+    int64_t run = ex + (offs ? offs[blockIdx.x] : 0);
+    #pragma unroll
+    for (int t = 0; t < SCAN_IPT; t++)
+    {
+        int64_t q = base + t;
+        if (q < n) out[q] = run;
+        run += v[t];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_NT - 1) out[n] = run;
+}
+
+int scan_exclusive_i64(const int64_t *d_in, int64_t *d_out, int64_t n)
+{
+    if (n <= 0)
+    {
+        GM_HIP(hipMemsetAsync(d_out, 0, sizeof(int64_t), g_stream), "scan memset");
+        return GM_SUCCESS;
+    }
+    int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt == 1)
+    {
+        ProfScope ps("scan_apply");
+        k_scan_apply<<<1, SCAN_NT, 0, g_stream>>>(d_in, n, nullptr, d_out);
+        return check_hip(hipGetLastError(), "k_scan_apply");
+    }
+    DevBuf sums;
+    if (!sums.alloc(sizeof(int64_t) * (nt + 1)))
+        return set_error(GM_OUT_OF_MEMORY, "scan workspace");
+    {
+        ProfScope ps("scan_reduce");
+        k_scan_reduce<<<(unsigned) nt, SCAN_NT, 0, g_stream>>>(d_in, n, sums.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_scan_reduce");
+    }
+    GM_TRY(scan_exclusive_i64(sums.as<int64_t>(), sums.as<int64_t>(), nt));
+    {
+        ProfScope ps("scan_apply");
+        k_scan_apply<<<(unsigned) nt, SCAN_NT, 0, g_stream>>>(d_in, n, sums.as<int64_t>(), d_out);
+        GM_HIP(hipGetLastError(), "k_scan_apply");
+    }
+    return GM_SUCCESS;
+}
+
+} // namespace gm
+
+using namespace gm;
+
+//------------------------------------------------------------------------------
+// C ABI
+//------------------------------------------------------------------------------
+
+extern "C" {
+
+int gm_init(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_init_rc == GM_SUCCESS) return GM_SUCCESS;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+    {
+        g_init_rc = set_error(GM_PANIC, "gm_init: no HIP device available (%s)",
+                              hipGetErrorString(e));
+        return g_init_rc;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (hipSetDevice(dev) != hipSuccess)
+    {
+        g_init_rc = set_error(GM_PANIC, "gm_init: hipSetDevice(%d) failed", dev);
+        return g_init_rc;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    {
+        g_init_rc = set_error(GM_PANIC, "gm_init: hipGetDeviceProperties failed");
+        return g_init_rc;
+    }
+    g_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
+    {
+        g_init_rc = set_error(GM_PANIC, "gm_init: stream creation failed");
+        return g_init_rc;
+    }
+    g_device = dev;
+    g_init_rc = GM_SUCCESS;
+    return GM_SUCCESS;
+}
+
+void gm_finalize(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_init_rc != GM_SUCCESS) return;
+    (void) hipStreamSynchronize(g_stream);
+    trim_locked();
+    for (auto &kv : g_live) (void) hipFree(kv.first);
+    g_live.clear();
+    g_live_bytes = 0;
+    for (auto &ev : g_prof_evs) { (void) hipEventDestroy(ev.a); (void) hipEventDestroy(ev.b); }
+    g_prof_evs.clear();
+    for (auto e : g_event_pool) (void) hipEventDestroy(e);
+    g_event_pool.clear();
+    if (g_pinned) (void) hipHostFree(g_pinned);
+    g_pinned = nullptr;
+    g_pinned_bytes = 0;
+    (void) hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+    g_init_rc = 1;
+}
+
+const char *gm_last_error(void) { return g_err; }
+
+int gm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gm_get_device_info(gm_device_info *info)
+{
+    GM_TRY(ensure_init());
+    if (!info) return GM_INVALID_VALUE;
+    hipDeviceProp_t prop;
+    GM_HIP(hipGetDeviceProperties(&prop, g_device), "hipGetDeviceProperties");
+    memset(info, 0, sizeof(*info));
+    snprintf(info->name, sizeof(info->name), "%s", prop.name);
+    snprintf(info->arch, sizeof(info->arch), "%s", prop.gcnArchName);
+    info->cu_count = prop.multiProcessorCount;
+    info->lds_per_block = (int32_t) prop.sharedMemPerBlock;
+    info->hbm_bytes = (int64_t) prop.totalGlobalMem;
+    // memoryClockRate is in kHz; double data rate
+    info->hbm_peak_gbs = 2.0 * (double) prop.memoryClockRate * 1e3 *
+                         ((double) prop.memoryBusWidth / 8.0) / 1e9;
+    return GM_SUCCESS;
+}
+
+void *gm_malloc(size_t bytes)
+{
+    if (ensure_init() != GM_SUCCESS) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_fail_after >= 0)
+    {
+        if (g_fail_after == 0)
+        {
+            g_fail_after = -1;
+            set_error(GM_OUT_OF_MEMORY, "gm_malloc: injected allocation failure");
+            return nullptr;
+        }
+        g_fail_after--;
+    }
+    size_t r = round_size(bytes);
+    auto it = g_free.lower_bound(r);
+    if (it != g_free.end() && it->first <= 2 * r)
+    {
+        void *p = it->second;
+        size_t sz = it->first;
+        g_free.erase(it);
+        g_cached_bytes -= sz;
+        g_live[p] = sz;
+        g_live_bytes += sz;
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess)
+    {
+        (void) hipGetLastError();
+        (void) hipStreamSynchronize(g_stream);
+        trim_locked();
+        e = hipMalloc(&p, r);
+        if (e != hipSuccess)
+        {
+            (void) hipGetLastError();
+            set_error(GM_OUT_OF_MEMORY, "gm_malloc: %zu bytes: %s", r, hipGetErrorString(e));
+            return nullptr;
+        }
+    }
+    g_live[p] = r;
+    g_live_bytes += r;
+    return p;
+}
+
+void gm_free(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) return;   // not ours: ignore
+    size_t sz = it->second;
+    g_live.erase(it);
+    g_live_bytes -= sz;
+    g_free.emplace(sz, p);
+    g_cached_bytes += sz;
+}
+
+int gm_arena_trim(void)
+{
+    if (g_init_rc != GM_SUCCESS) return GM_SUCCESS;
+    std::lock_guard<std::mutex> lk(g_mu);
+    (void) hipStreamSynchronize(g_stream);
+    trim_locked();
+    return GM_SUCCESS;
+}
+
+int64_t gm_arena_bytes_in_use(void) { return (int64_t) g_live_bytes; }
+
+void gm_debug_fail_after(int64_t n)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_fail_after = n;
+}
+
+int gm_memcpy_h2d(void *dst, const void *src, size_t bytes)
+{
+    GM_TRY(ensure_init());
+    if (bytes == 0) return GM_SUCCESS;
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream), "h2d");
+    GM_HIP(hipStreamSynchronize(g_stream), "h2d sync");
+    return GM_SUCCESS;
+}
+
+int gm_memcpy_d2h(void *dst, const void *src, size_t bytes)
+{
+    GM_TRY(ensure_init());
+    if (bytes == 0) return GM_SUCCESS;
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream), "d2h");
+    GM_HIP(hipStreamSynchronize(g_stream), "d2h sync");
+    return GM_SUCCESS;
+}
+
+int gm_memcpy_d2d(void *dst, const void *src, size_t bytes)
+{
+    GM_TRY(ensure_init());
+    if (bytes == 0) return GM_SUCCESS;
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream), "d2d");
+    return GM_SUCCESS;
+}
+
+int gm_memset(void *dst, int value, size_t bytes)
+{
+    GM_TRY(ensure_init());
+    if (bytes == 0) return GM_SUCCESS;
+    GM_HIP(hipMemsetAsync(dst, value, bytes, g_stream), "memset");
+    return GM_SUCCESS;
+}
+
+int gm_synchronize(void)
+{
+    GM_TRY(ensure_init());
+    GM_HIP(hipStreamSynchronize(g_stream), "synchronize");
+    return GM_SUCCESS;
+}
+
+void gm_prof_enable(int on) { g_prof = (on != 0); }
+
+void gm_prof_reset(void)
+{
+    if (g_init_rc == GM_SUCCESS) (void) hipStreamSynchronize(g_stream);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &ev : g_prof_evs) { g_event_pool.push_back(ev.a); g_event_pool.push_back(ev.b); }
+    g_prof_evs.clear();
+    g_prof_names.clear();
+}
+
+int gm_prof_get(gm_prof_record *rec, int cap)
+{
+    if (g_init_rc == GM_SUCCESS) (void) hipStreamSynchronize(g_stream);
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::vector<double> tot(g_prof_names.size(), 0.0);
+    std::vector<int64_t> cnt(g_prof_names.size(), 0);
+    for (auto &ev : g_prof_evs)
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev.a, ev.b) == hipSuccess)
+        {
+            tot[ev.name_id] += ms;
+            cnt[ev.name_id]++;
+        }
+    }
+    int n = (int) g_prof_names.size();
+    for (int t = 0; t < n && t < cap; t++)
+    {
+        memset(&rec[t], 0, sizeof(rec[t]));
+        snprintf(rec[t].name, sizeof(rec[t].name), "%s", g_prof_names[t].c_str());
+        rec[t].launches = cnt[t];
+        rec[t].total_ms = tot[t];
+    }
+    return n;
+}
+
+} // extern "C"

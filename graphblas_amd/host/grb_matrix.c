@@ -1,0 +1,555 @@
+// grb_matrix.c: CSR matrices held in HBM (new/free/dup/build/extract/pack).
+//
+// Construction and extraction run on the host (sort + combine of tuples, as
+// GrB_Matrix_build does on the CPU in the reference, Source/builder) and move
+// the CSR arrays across PCIe once; every GrB_mxm then works on the
+// device-resident arrays.
+
+#include "grb_internal.h"
+
+GrB_Info GB_check_matrix(const GrB_Matrix A)
+{
+    if (!A) return GrB_NULL_POINTER;
+    if (A->magic == GB_FREED) return GrB_INVALID_OBJECT;
+    if (A->magic != GB_MAGIC) return GrB_UNINITIALIZED_OBJECT;
+    return GrB_SUCCESS;
+}
+
+void GB_matrix_free_content(GrB_Matrix A)
+{
+    gm_free(A->p);
+    gm_free(A->j);
+    gm_free(A->x);
+    A->p = NULL;
+    A->j = NULL;
+    A->x = NULL;
+    A->nvals = 0;
+    A->iso = false;
+}
+
+GrB_Info GB_matrix_set_csr(GrB_Matrix A, const gm_csr_out *R)
+{
+    GB_matrix_free_content(A);
+    A->p = R->p;
+    A->j = R->j;
+    A->x = R->x;
+    A->nvals = R->nvals;
+    A->iso = R->iso != 0;
+    return GrB_SUCCESS;
+}
+
+// Device view of A for the kernel layer; an empty matrix gets zeroed row
+// pointers on first use.
+GrB_Info GB_matrix_view(gm_csr *v, GrB_Matrix A)
+{
+    if (A->p == NULL)
+    {
+        size_t bytes = sizeof(int64_t) * (size_t) (A->nrows + 1);
+        A->p = (int64_t *) gm_malloc(bytes);
+        if (!A->p) return GB_error(A, GB_from_gm(gm_init()) == GrB_SUCCESS ? GrB_OUT_OF_MEMORY
+                                                                          : GrB_PANIC,
+                                   "%s", gm_last_error());
+        int rc = gm_memset(A->p, 0, bytes);
+        if (rc != GM_SUCCESS) return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+        A->nvals = 0;
+    }
+    v->nrows = A->nrows;
+    v->ncols = A->ncols;
+    v->p = A->p;
+    v->j = A->j;
+    v->x = A->x;
+    v->iso = A->iso ? 1 : 0;
+    v->type = A->type->code;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_new(GrB_Matrix *A, GrB_Type type, GrB_Index nrows, GrB_Index ncols)
+{
+    if (!GB_initialized()) return GrB_PANIC;
+    if (!A || !type) return GrB_NULL_POINTER;
+    *A = NULL;
+    if (type->magic != GB_MAGIC) return GrB_UNINITIALIZED_OBJECT;
+    if (nrows > GrB_INDEX_MAX || ncols > GrB_INDEX_MAX) return GrB_INVALID_VALUE;
+    GrB_Matrix M = (GrB_Matrix) calloc(1, sizeof(struct GB_Matrix_opaque));
+    if (!M) return GrB_OUT_OF_MEMORY;
+    M->magic = GB_MAGIC;
+    M->type = type;
+    M->nrows = (int64_t) nrows;
+    M->ncols = (int64_t) ncols;
+    *A = M;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_free(GrB_Matrix *A)
+{
+    if (!A || !*A) return GrB_SUCCESS;
+    GrB_Matrix M = *A;
+    if (M->magic == GB_MAGIC)
+    {
+        GB_matrix_free_content(M);
+        M->magic = GB_FREED;
+        free(M);
+    }
+    *A = NULL;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_clear(GrB_Matrix A)
+{
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    GB_matrix_free_content(A);
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_nrows(GrB_Index *nrows, const GrB_Matrix A)
+{
+    if (!nrows) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    *nrows = (GrB_Index) A->nrows;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_ncols(GrB_Index *ncols, const GrB_Matrix A)
+{
+    if (!ncols) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    *ncols = (GrB_Index) A->ncols;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_nvals(GrB_Index *nvals, const GrB_Matrix A)
+{
+    if (!nvals) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    *nvals = (GrB_Index) A->nvals;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_type(GrB_Type *type, const GrB_Matrix A)
+{
+    if (!type) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    *type = A->type;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_iso(bool *iso, const GrB_Matrix A)
+{
+    if (!iso) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    *iso = A->iso;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_dup(GrB_Matrix *C, const GrB_Matrix A)
+{
+    if (!C) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    info = GrB_Matrix_new(C, A->type, (GrB_Index) A->nrows, (GrB_Index) A->ncols);
+    if (info != GrB_SUCCESS) return info;
+    GrB_Matrix D = *C;
+    if (A->p == NULL) return GrB_SUCCESS;
+    size_t pb = sizeof(int64_t) * (size_t) (A->nrows + 1);
+    size_t jb = sizeof(int64_t) * (size_t) (A->nvals > 0 ? A->nvals : 1);
+    size_t xb = A->type->size * (size_t) (A->iso ? 1 : (A->nvals > 0 ? A->nvals : 1));
+    D->p = (int64_t *) gm_malloc(pb);
+    D->j = (int64_t *) gm_malloc(jb);
+    D->x = A->x ? gm_malloc(xb) : NULL;
+    if (!D->p || !D->j || (A->x && !D->x))
+    {
+        GrB_Matrix_free(C);
+        return GrB_OUT_OF_MEMORY;
+    }
+    int rc = gm_memcpy_d2d(D->p, A->p, pb);
+    if (rc == GM_SUCCESS && A->j) rc = gm_memcpy_d2d(D->j, A->j, jb);
+    if (rc == GM_SUCCESS && A->x) rc = gm_memcpy_d2d(D->x, A->x, xb);
+    if (rc == GM_SUCCESS) rc = gm_synchronize();
+    if (rc != GM_SUCCESS)
+    {
+        GrB_Matrix_free(C);
+        return GB_from_gm(rc);
+    }
+    D->nvals = A->nvals;
+    D->iso = A->iso;
+    return GrB_SUCCESS;
+}
+
+//------------------------------------------------------------------------------
+// upload / download helpers
+//------------------------------------------------------------------------------
+
+// upload host CSR (sorted rows, no duplicates) into A
+static GrB_Info GB_upload(GrB_Matrix A, const int64_t *Ap, const int64_t *Aj, const void *Ax,
+                          int64_t nvals, bool iso)
+{
+    size_t pb = sizeof(int64_t) * (size_t) (A->nrows + 1);
+    size_t jb = sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1);
+    size_t xb = A->type->size * (size_t) (iso ? 1 : (nvals > 0 ? nvals : 1));
+    int64_t *dp = (int64_t *) gm_malloc(pb);
+    int64_t *dj = (int64_t *) gm_malloc(jb);
+    void *dx = gm_malloc(xb);
+    if (!dp || !dj || !dx)
+    {
+        gm_free(dp);
+        gm_free(dj);
+        gm_free(dx);
+        int rc = gm_init();
+        return GB_error(A, rc == GM_SUCCESS ? GrB_OUT_OF_MEMORY : GrB_PANIC, "%s", gm_last_error());
+    }
+    int rc = gm_memcpy_h2d(dp, Ap, pb);
+    if (rc == GM_SUCCESS && nvals > 0) rc = gm_memcpy_h2d(dj, Aj, sizeof(int64_t) * (size_t) nvals);
+    if (rc == GM_SUCCESS && Ax && (nvals > 0 || iso))
+        rc = gm_memcpy_h2d(dx, Ax, A->type->size * (size_t) (iso ? 1 : nvals));
+    if (rc == GM_SUCCESS && !Ax) rc = gm_memset(dx, 0, xb);
+    if (rc != GM_SUCCESS)
+    {
+        gm_free(dp);
+        gm_free(dj);
+        gm_free(dx);
+        return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+    }
+    GB_matrix_free_content(A);
+    A->p = dp;
+    A->j = dj;
+    A->x = dx;
+    A->nvals = nvals;
+    A->iso = iso;
+    return GrB_SUCCESS;
+}
+
+// download A into malloc'd host arrays
+static GrB_Info GB_download(GrB_Matrix A, int64_t **Ap, int64_t **Aj, void **Ax)
+{
+    int64_t n = A->nrows, nz = A->nvals;
+    size_t xsz = A->type->size * (size_t) (A->iso ? 1 : (nz > 0 ? nz : 1));
+    int64_t *hp = (int64_t *) malloc(sizeof(int64_t) * (size_t) (n + 1));
+    int64_t *hj = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nz > 0 ? nz : 1));
+    void *hx = malloc(xsz);
+    if (!hp || !hj || !hx)
+    {
+        free(hp);
+        free(hj);
+        free(hx);
+        return GrB_OUT_OF_MEMORY;
+    }
+    if (A->p == NULL)
+    {
+        memset(hp, 0, sizeof(int64_t) * (size_t) (n + 1));
+        memset(hx, 0, xsz);
+    }
+    else
+    {
+        int rc = gm_memcpy_d2h(hp, A->p, sizeof(int64_t) * (size_t) (n + 1));
+        if (rc == GM_SUCCESS && nz > 0) rc = gm_memcpy_d2h(hj, A->j, sizeof(int64_t) * (size_t) nz);
+        if (rc == GM_SUCCESS && A->x && (nz > 0 || A->iso)) rc = gm_memcpy_d2h(hx, A->x, xsz);
+        if (rc != GM_SUCCESS)
+        {
+            free(hp);
+            free(hj);
+            free(hx);
+            return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+        }
+    }
+    *Ap = hp;
+    *Aj = hj;
+    *Ax = hx;
+    return GrB_SUCCESS;
+}
+
+//------------------------------------------------------------------------------
+// build from tuples
+//------------------------------------------------------------------------------
+
+typedef struct { uint64_t i, j, k; } GB_tuple;
+
+static int GB_tuple_cmp(const void *a, const void *b)
+{
+    const GB_tuple *x = (const GB_tuple *) a, *y = (const GB_tuple *) b;
+    if (x->i != y->i) return x->i < y->i ? -1 : 1;
+    if (x->j != y->j) return x->j < y->j ? -1 : 1;
+    return x->k < y->k ? -1 : (x->k > y->k);   // keep input order of duplicates
+}
+
+// z = dup(x, y) for the built-in operators; false if not applicable
+static bool GB_apply_dup(const GrB_BinaryOp op, int32_t code, void *z, const void *x, const void *y)
+{
+#define GB_DUP_CASE(CODE, CT)                                            \
+    if (code == CODE)                                                    \
+    {                                                                    \
+        CT a = *(const CT *) x, b = *(const CT *) y, r;                  \
+        switch (op->opcode)                                              \
+        {                                                                \
+            case GB_OP_PLUS: r = (CT) (a + b); break;                    \
+            case GB_OP_MINUS: r = (CT) (a - b); break;                   \
+            case GB_OP_TIMES: r = (CT) (a * b); break;                   \
+            case GB_OP_MIN: r = a < b ? a : b; break;                    \
+            case GB_OP_MAX: r = a > b ? a : b; break;                    \
+            case GB_OP_FIRST: case GB_OP_ANY: r = a; break;              \
+            case GB_OP_SECOND: r = b; break;                             \
+            case GB_OP_PAIR: r = (CT) 1; break;                          \
+            case GB_OP_LOR: r = (CT) (a || b); break;                    \
+            case GB_OP_LAND: r = (CT) (a && b); break;                   \
+            default: return false;                                       \
+        }                                                                \
+        *(CT *) z = r;                                                   \
+        return true;                                                     \
+    }
+    GB_DUP_CASE(GM_BOOL, bool)
+    GB_DUP_CASE(GM_INT32, int32_t)
+    GB_DUP_CASE(GM_INT64, int64_t)
+    GB_DUP_CASE(GM_FP64, double)
+#undef GB_DUP_CASE
+    return false;
+}
+
+static GrB_Info GB_build(GrB_Matrix C, const GrB_Index *I, const GrB_Index *J, const void *X,
+                         GrB_Index nvals, const GrB_BinaryOp dup, int32_t xcode)
+{
+    GrB_Info info = GB_check_matrix(C);
+    if (info != GrB_SUCCESS) return info;
+    if ((!I || !J || !X) && nvals > 0) return GrB_NULL_POINTER;
+    if (C->nvals > 0) return GB_error(C, GrB_OUTPUT_NOT_EMPTY, "GrB_Matrix_build: C not empty");
+    if (C->type->code != xcode)
+        return GB_error(C, GrB_DOMAIN_MISMATCH, "GrB_Matrix_build: X type differs from C type");
+    if (dup && dup->magic != GB_MAGIC) return GrB_UNINITIALIZED_OBJECT;
+    const size_t xs = C->type->size;
+    GB_tuple *t = (GB_tuple *) malloc(sizeof(GB_tuple) * (size_t) (nvals > 0 ? nvals : 1));
+    int64_t *Ap = (int64_t *) calloc((size_t) C->nrows + 1, sizeof(int64_t));
+    int64_t *Aj = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1));
+    uint8_t *Ax = (uint8_t *) malloc(xs * (size_t) (nvals > 0 ? nvals : 1));
+    if (!t || !Ap || !Aj || !Ax)
+    {
+        free(t); free(Ap); free(Aj); free(Ax);
+        return GrB_OUT_OF_MEMORY;
+    }
+    for (GrB_Index k = 0; k < nvals; k++)
+    {
+        if (I[k] >= (GrB_Index) C->nrows || J[k] >= (GrB_Index) C->ncols)
+        {
+            free(t); free(Ap); free(Aj); free(Ax);
+            return GB_error(C, GrB_INDEX_OUT_OF_BOUNDS, "GrB_Matrix_build: tuple %llu out of bounds",
+                            (unsigned long long) k);
+        }
+        t[k].i = I[k];
+        t[k].j = J[k];
+        t[k].k = k;
+    }
+    qsort(t, (size_t) nvals, sizeof(GB_tuple), GB_tuple_cmp);
+    int64_t nz = 0;
+    for (GrB_Index k = 0; k < nvals; k++)
+    {
+        const uint8_t *xk = (const uint8_t *) X + xs * t[k].k;
+        if (nz > 0 && k > 0 && t[k].i == t[k - 1].i && t[k].j == t[k - 1].j)
+        {
+            if (!dup)
+            {
+                free(t); free(Ap); free(Aj); free(Ax);
+                return GB_error(C, GrB_INVALID_VALUE, "GrB_Matrix_build: duplicate without dup op");
+            }
+            if (!GB_apply_dup(dup, xcode, Ax + xs * (nz - 1), Ax + xs * (nz - 1), xk))
+            {
+                free(t); free(Ap); free(Aj); free(Ax);
+                return GB_error(C, GrB_DOMAIN_MISMATCH, "GrB_Matrix_build: dup op not supported");
+            }
+            continue;
+        }
+        Aj[nz] = (int64_t) t[k].j;
+        memcpy(Ax + xs * nz, xk, xs);
+        Ap[t[k].i + 1]++;
+        nz++;
+    }
+    for (int64_t i = 0; i < C->nrows; i++) Ap[i + 1] += Ap[i];
+    free(t);
+    info = GB_upload(C, Ap, Aj, Ax, nz, false);
+    free(Ap);
+    free(Aj);
+    free(Ax);
+    return info;
+}
+
+GrB_Info GrB_Matrix_build_BOOL(GrB_Matrix C, const GrB_Index *I, const GrB_Index *J,
+                               const bool *X, GrB_Index nvals, const GrB_BinaryOp dup)
+{
+    return GB_build(C, I, J, X, nvals, dup, GM_BOOL);
+}
+GrB_Info GrB_Matrix_build_INT32(GrB_Matrix C, const GrB_Index *I, const GrB_Index *J,
+                                const int32_t *X, GrB_Index nvals, const GrB_BinaryOp dup)
+{
+    return GB_build(C, I, J, X, nvals, dup, GM_INT32);
+}
+GrB_Info GrB_Matrix_build_INT64(GrB_Matrix C, const GrB_Index *I, const GrB_Index *J,
+                                const int64_t *X, GrB_Index nvals, const GrB_BinaryOp dup)
+{
+    return GB_build(C, I, J, X, nvals, dup, GM_INT64);
+}
+GrB_Info GrB_Matrix_build_FP64(GrB_Matrix C, const GrB_Index *I, const GrB_Index *J,
+                               const double *X, GrB_Index nvals, const GrB_BinaryOp dup)
+{
+    return GB_build(C, I, J, X, nvals, dup, GM_FP64);
+}
+
+//------------------------------------------------------------------------------
+// extract tuples
+//------------------------------------------------------------------------------
+
+static GrB_Info GB_extract(GrB_Index *I, GrB_Index *J, void *X, GrB_Index *nvals,
+                           const GrB_Matrix A, int32_t xcode)
+{
+    if (!nvals) return GrB_NULL_POINTER;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (X && A->type->code != xcode)
+        return GB_error(A, GrB_DOMAIN_MISMATCH, "extractTuples: X type differs from A type");
+    if (*nvals < (GrB_Index) A->nvals)
+        return GB_error(A, GrB_INSUFFICIENT_SPACE, "extractTuples: arrays too small");
+    int64_t *Ap, *Aj;
+    void *Ax;
+    info = GB_download(A, &Ap, &Aj, &Ax);
+    if (info != GrB_SUCCESS) return info;
+    const size_t xs = A->type->size;
+    for (int64_t i = 0; i < A->nrows; i++)
+    {
+        for (int64_t p = Ap[i]; p < Ap[i + 1]; p++)
+        {
+            if (I) I[p] = (GrB_Index) i;
+            if (J) J[p] = (GrB_Index) Aj[p];
+            if (X) memcpy((uint8_t *) X + xs * p, (uint8_t *) Ax + xs * (A->iso ? 0 : p), xs);
+        }
+    }
+    *nvals = (GrB_Index) A->nvals;
+    free(Ap);
+    free(Aj);
+    free(Ax);
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_extractTuples_BOOL(GrB_Index *I, GrB_Index *J, bool *X, GrB_Index *nvals,
+                                       const GrB_Matrix A)
+{
+    return GB_extract(I, J, X, nvals, A, GM_BOOL);
+}
+GrB_Info GrB_Matrix_extractTuples_INT32(GrB_Index *I, GrB_Index *J, int32_t *X,
+                                        GrB_Index *nvals, const GrB_Matrix A)
+{
+    return GB_extract(I, J, X, nvals, A, GM_INT32);
+}
+GrB_Info GrB_Matrix_extractTuples_INT64(GrB_Index *I, GrB_Index *J, int64_t *X,
+                                        GrB_Index *nvals, const GrB_Matrix A)
+{
+    return GB_extract(I, J, X, nvals, A, GM_INT64);
+}
+GrB_Info GrB_Matrix_extractTuples_FP64(GrB_Index *I, GrB_Index *J, double *X,
+                                       GrB_Index *nvals, const GrB_Matrix A)
+{
+    return GB_extract(I, J, X, nvals, A, GM_FP64);
+}
+
+//------------------------------------------------------------------------------
+// pack / unpack CSR
+//------------------------------------------------------------------------------
+
+static int GB_i64_cmp(const void *a, const void *b)
+{
+    int64_t x = *(const int64_t *) a, y = *(const int64_t *) b;
+    return (x > y) - (x < y);
+}
+
+GrB_Info GxB_Matrix_pack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, void **Ax,
+                             GrB_Index Ap_size, GrB_Index Aj_size, GrB_Index Ax_size,
+                             bool iso, bool jumbled, const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!Ap || !Aj || !*Ap) return GrB_NULL_POINTER;
+    const int64_t n = A->nrows;
+    if (Ap_size < sizeof(int64_t) * (GrB_Index) (n + 1))
+        return GB_error(A, GrB_INVALID_VALUE, "pack_CSR: Ap too small");
+    int64_t *p = (int64_t *) *Ap;
+    int64_t *j = (int64_t *) (*Aj);
+    const int64_t nz = p[n];
+    if (nz < 0 || Aj_size < sizeof(int64_t) * (GrB_Index) nz)
+        return GB_error(A, GrB_INVALID_VALUE, "pack_CSR: Aj too small");
+    if (nz > 0 && !j) return GrB_NULL_POINTER;
+    const size_t xs = A->type->size;
+    const void *x = Ax ? *Ax : NULL;
+    if (x && Ax_size < xs * (GrB_Index) (iso ? 1 : nz))
+        return GB_error(A, GrB_INVALID_VALUE, "pack_CSR: Ax too small");
+    if (!x && (nz > 0 || iso)) return GrB_NULL_POINTER;
+    if (jumbled)
+    {
+        // the kernels need sorted rows; sort each row (with its values)
+        for (int64_t i = 0; i < n; i++)
+        {
+            int64_t len = p[i + 1] - p[i];
+            if (len < 2) continue;
+            if (iso)
+            {
+                qsort(j + p[i], (size_t) len, sizeof(int64_t), GB_i64_cmp);
+            }
+            else
+            {
+                // insertion sort keeps values paired; rows are short in practice
+                uint8_t *xv = (uint8_t *) x;
+                uint8_t tmp[16];
+                for (int64_t a = p[i] + 1; a < p[i + 1]; a++)
+                {
+                    int64_t key = j[a];
+                    memcpy(tmp, xv + xs * a, xs);
+                    int64_t b = a - 1;
+                    while (b >= p[i] && j[b] > key)
+                    {
+                        j[b + 1] = j[b];
+                        memcpy(xv + xs * (b + 1), xv + xs * b, xs);
+                        b--;
+                    }
+                    j[b + 1] = key;
+                    memcpy(xv + xs * (b + 1), tmp, xs);
+                }
+            }
+        }
+    }
+    info = GB_upload(A, p, j, x, nz, iso);
+    if (info != GrB_SUCCESS) return info;
+    free(*Ap);
+    *Ap = NULL;
+    free(*Aj);
+    *Aj = NULL;
+    if (Ax)
+    {
+        free(*Ax);
+        *Ax = NULL;
+    }
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, void **Ax,
+                               GrB_Index *Ap_size, GrB_Index *Aj_size, GrB_Index *Ax_size,
+                               bool *iso, bool *jumbled, const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!Ap || !Aj || !Ax || !Ap_size || !Aj_size || !Ax_size) return GrB_NULL_POINTER;
+    int64_t *hp, *hj;
+    void *hx;
+    info = GB_download(A, &hp, &hj, &hx);
+    if (info != GrB_SUCCESS) return info;
+    *Ap = (GrB_Index *) hp;
+    *Aj = (GrB_Index *) hj;
+    *Ax = hx;
+    *Ap_size = sizeof(int64_t) * (GrB_Index) (A->nrows + 1);
+    *Aj_size = sizeof(int64_t) * (GrB_Index) (A->nvals > 0 ? A->nvals : 1);
+    *Ax_size = A->type->size * (GrB_Index) (A->iso ? 1 : (A->nvals > 0 ? A->nvals : 1));
+    if (iso) *iso = A->iso;
+    if (jumbled) *jumbled = false;
+    GB_matrix_free_content(A);
+    return GrB_SUCCESS;
+}

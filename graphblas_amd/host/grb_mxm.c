@@ -1,0 +1,137 @@
+// grb_mxm.c: GrB_mxm, C<Mask> = accum (C, A*B), on the gfx950 kernels.
+//
+// Follows the reference's call chain for CSR operands:
+//   GrB_mxm      Source/mxm/GrB_mxm.c:18-67   (checks, descriptor decode)
+//   GB_mxm       Source/mxm/GB_mxm.c:24-222   (types, dimensions, transplant)
+//   GB_AxB_meta  Source/mxm/GB_AxB_meta.c:41-875 (orientation, method choice)
+// With every matrix stored by row, the reference's CSR folding and swap rule
+// (GB_AxB_meta.c:214-447) end in two kernels without explicit transposes:
+//   C<M> = A*B'  with M present and not complemented  -> dot3
+//                (GB_AxB_dot3_control, Source/mxm/GB_mxm.h:233-241)
+//   C = A*B, C<M> = A*B, C<!M> = A*B                   -> saxpy3
+//                (GB_AxB_saxpy_sparsity.c:56-83, default policy)
+// Other shapes (A' operands, an unmasked or complemented A*B', accum, or a
+// masked update of a non-empty C without GrB_REPLACE) return
+// GrB_NOT_IMPLEMENTED: there is no CPU path behind this library.
+
+#include "grb_internal.h"
+
+GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran, int axb)
+{
+    (void) axb;
+    if (a_tran) return GB_METHOD_NONE;
+    if (b_tran) return (has_mask && !mask_comp) ? GB_METHOD_DOT3 : GB_METHOD_NONE;
+    return GB_METHOD_SAXPY3;
+}
+
+static const char *GB_sr_name(const gm_semiring *s, char *buf, size_t len)
+{
+    static const char *adds[] = {"?", "plus", "min", "max", "any"};
+    static const char *mults[] = {"?", "times", "plus", "pair", "k"};
+    static const char *types[] = {"?", "bool", "int8", "uint8", "int16", "uint16", "int32",
+                                  "uint32", "int64", "uint64", "fp32", "fp64"};
+    snprintf(buf, len, "%s_%s_%s", adds[(s->add_op >= 0 && s->add_op < 5) ? s->add_op : 0],
+             mults[(s->mult_op >= 0 && s->mult_op < 5) ? s->mult_op : 0],
+             types[(s->ztype >= 0 && s->ztype < 12) ? s->ztype : 0]);
+    return buf;
+}
+
+GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
+                 const GrB_Semiring semiring, const GrB_Matrix A, const GrB_Matrix B,
+                 const GrB_Descriptor desc)
+{
+    if (!GB_initialized()) return GrB_PANIC;
+    GrB_Info info;
+    if ((info = GB_check_matrix(C)) != GrB_SUCCESS) return info;
+    if (!semiring) return GB_error(C, GrB_NULL_POINTER, "GrB_mxm: semiring is NULL");
+    if (semiring->magic != GB_MAGIC)
+        return GB_error(C, GrB_UNINITIALIZED_OBJECT, "GrB_mxm: semiring not initialised");
+    if ((info = GB_check_matrix(A)) != GrB_SUCCESS) return GB_error(C, info, "GrB_mxm: A invalid");
+    if ((info = GB_check_matrix(B)) != GrB_SUCCESS) return GB_error(C, info, "GrB_mxm: B invalid");
+    if (Mask && (info = GB_check_matrix(Mask)) != GrB_SUCCESS)
+        return GB_error(C, info, "GrB_mxm: Mask invalid");
+    if (accum && accum->magic != GB_MAGIC)
+        return GB_error(C, GrB_UNINITIALIZED_OBJECT, "GrB_mxm: accum not initialised");
+    if (desc && desc->magic != GB_MAGIC)
+        return GB_error(C, GrB_UNINITIALIZED_OBJECT, "GrB_mxm: descriptor not initialised");
+
+    // descriptor decode (GrB_mxm.c:43-47)
+    const bool C_replace = desc && desc->out == GrB_REPLACE;
+    const bool mask_comp = desc && (desc->mask & GrB_COMP);
+    const bool mask_struct = desc && (desc->mask & GrB_STRUCTURE);
+    const bool a_tran = desc && desc->in0 == GrB_TRAN;
+    const bool b_tran = desc && desc->in1 == GrB_TRAN;
+    const int axb = desc ? desc->axb : 0;
+
+    // dimensions (GB_mxm.c:85-99)
+    const int64_t am = a_tran ? A->ncols : A->nrows, an = a_tran ? A->nrows : A->ncols;
+    const int64_t bm = b_tran ? B->ncols : B->nrows, bn = b_tran ? B->nrows : B->ncols;
+    if (an != bm || C->nrows != am || C->ncols != bn)
+        return GB_error(C, GrB_DIMENSION_MISMATCH,
+                        "GrB_mxm: dimensions (%lld x %lld) = (%lld x %lld) * (%lld x %lld)",
+                        (long long) C->nrows, (long long) C->ncols, (long long) am,
+                        (long long) an, (long long) bm, (long long) bn);
+    if (Mask && (Mask->nrows != C->nrows || Mask->ncols != C->ncols))
+        return GB_error(C, GrB_DIMENSION_MISMATCH, "GrB_mxm: Mask is %lld x %lld, C is %lld x %lld",
+                        (long long) Mask->nrows, (long long) Mask->ncols, (long long) C->nrows,
+                        (long long) C->ncols);
+
+    // types (GB_mxm.c:65-82): the semiring result must be C's type here
+    gm_semiring sr;
+    if (!GB_semiring_codes(semiring, &sr))
+        return GB_error(C, GrB_NOT_IMPLEMENTED, "GrB_mxm: semiring %s not supported",
+                        semiring->name);
+    if (C->type->code != sr.ztype)
+        return GB_error(C, GrB_NOT_IMPLEMENTED,
+                        "GrB_mxm: typecast of the %s result to C's type %s is not supported",
+                        semiring->name, C->type->name);
+    const bool valued = (sr.mult_op == GM_MULT_TIMES || sr.mult_op == GM_MULT_PLUS);
+    if (valued && (A->type->code != sr.ztype || B->type->code != sr.ztype))
+        return GB_error(C, GrB_NOT_IMPLEMENTED,
+                        "GrB_mxm: A and B must have the semiring's type %s", C->type->name);
+    if (accum)
+        return GB_error(C, GrB_NOT_IMPLEMENTED, "GrB_mxm: accum is not supported");
+    if (Mask && !C_replace && C->nvals > 0)
+        return GB_error(C, GrB_NOT_IMPLEMENTED,
+                        "GrB_mxm: masked update of a non-empty C needs GrB_REPLACE");
+
+    const GB_method method = GB_mxm_select(Mask != NULL, mask_comp, a_tran, b_tran, axb);
+    if (method == GB_METHOD_NONE)
+        return GB_error(C, GrB_NOT_IMPLEMENTED,
+                        "GrB_mxm: operand orientation needs an explicit transpose (not supported)");
+
+    gm_csr va, vb, vm;
+    if ((info = GB_matrix_view(&va, A)) != GrB_SUCCESS) return GB_error(C, info, "%s", A->err);
+    if ((info = GB_matrix_view(&vb, B)) != GrB_SUCCESS) return GB_error(C, info, "%s", B->err);
+    if (Mask && (info = GB_matrix_view(&vm, Mask)) != GrB_SUCCESS)
+        return GB_error(C, info, "%s", Mask->err);
+    // a valued boolean mask of an iso matrix with a NULL value array counts
+    // as all true (structural)
+    const bool mstruct = mask_struct || (Mask && Mask->x == NULL);
+
+    gm_csr_out R;
+    memset(&R, 0, sizeof(R));
+    int rc;
+    char nm[64];
+    if (method == GB_METHOD_DOT3)
+    {
+        if (GB_burble)
+            printf("burble: GrB_mxm C<%sM>=A*B' %s: dot3 (masked dot product), M nnz %lld\n",
+                   mstruct ? "S:" : "", GB_sr_name(&sr, nm, sizeof(nm)),
+                   (long long) Mask->nvals);
+        rc = gm_dot3(&R, &vm, mstruct, &va, &vb, &sr, NULL);
+    }
+    else
+    {
+        if (GB_burble)
+            printf("burble: GrB_mxm C%s=A*B %s: saxpy3 (hash), A nnz %lld B nnz %lld\n",
+                   Mask ? (mask_comp ? "<!M>" : "<M>") : "", GB_sr_name(&sr, nm, sizeof(nm)),
+                   (long long) A->nvals, (long long) B->nvals);
+        rc = gm_saxpy3(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, NULL);
+    }
+    if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+    // transplant T into C (GB_mxm.c:168-200)
+    GB_matrix_set_csr(C, &R);
+    C->err[0] = '\0';
+    return GrB_SUCCESS;
+}

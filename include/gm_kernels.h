@@ -1,0 +1,196 @@
+//------------------------------------------------------------------------------
+// gm_kernels.h: C-ABI of the MI355X (gfx950) kernel layer behind GrB_mxm
+//------------------------------------------------------------------------------
+
+// This is the thin layer between the C host library (GrB_mxm and friends,
+// include/GraphBLAS.h) and the hand-written HIP kernels.  Every entry point is
+// plain C: device pointers, sizes and small POD structs.  All device memory
+// named here lives in HBM of the current HIP device; the library owns one
+// non-blocking stream (default priority) and a caching device arena.
+//
+// Reference interfaces replaced (SuiteSparse:GraphBLAS v9.3.1):
+//   gm_dot3_branch / gm_dot3     <- GB_cuda_AxB_dot3_branch / GB_cuda_AxB_dot3
+//                                   (Source/gateway/GB_cuda_gateway.h:237-255,
+//                                   called from Source/mxm/GB_AxB_dot.c:189-197)
+//   gm_saxpy3_branch / gm_saxpy3 <- GB_AxB_saxpy3 (Source/mxm/GB_AxB_saxpy3.c
+//                                   :119-700; the reference has no GPU saxpy3
+//                                   hook, so this mirrors the dot3 gateway pair)
+//   gm_reduce_scalar             <- GB_reduce_to_scalar
+//                                   (Source/reduce/template/
+//                                   GB_reduce_to_scalar_template.c:40-59)
+//   gm_init / gm_finalize        <- GB_cuda_init (CUDA/GB_cuda_init.c) and the
+//                                   rmm pool set-up in Source/init/GB_init.c:84-95
+//   gm_malloc / gm_free          <- rmm_wrap_malloc / rmm_wrap_free
+//                                   (rmm_wrap/rmm_wrap.h)
+//
+// Return codes are GrB_Info values (Include/GraphBLAS.h:358-381): 0 success,
+// 1 (GrB_NO_VALUE) "declined, try another implementation" as in
+// Source/mxm/GB_AxB_dot3.c:277-336, negative values are errors.
+
+#ifndef GM_KERNELS_H
+#define GM_KERNELS_H
+
+#include <stdint.h>
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// status codes (GrB_Info numbering)
+#define GM_SUCCESS            0
+#define GM_NO_VALUE           1
+#define GM_INVALID_VALUE     (-3)
+#define GM_NOT_IMPLEMENTED   (-8)
+#define GM_PANIC             (-101)
+#define GM_OUT_OF_MEMORY     (-102)
+
+// type codes: same order as the reference's GB_Type_code
+// (Source/builtin/include/GB_opaque.h, GB_BOOL_code .. GB_FP64_code)
+typedef enum
+{
+    GM_BOOL = 1, GM_INT8 = 2, GM_UINT8 = 3, GM_INT16 = 4, GM_UINT16 = 5,
+    GM_INT32 = 6, GM_UINT32 = 7, GM_INT64 = 8, GM_UINT64 = 9,
+    GM_FP32 = 10, GM_FP64 = 11
+} gm_type;
+
+// monoid ("add") and multiplier codes of a built-in semiring, as the
+// reference decodes them in GB_AxB_semiring_builtin
+// (Source/mxm/GB_AxB_semiring_builtin.c:20-123)
+typedef enum { GM_ADD_PLUS = 1, GM_ADD_MIN = 2, GM_ADD_MAX = 3, GM_ADD_ANY = 4 } gm_add_op;
+typedef enum
+{
+    GM_MULT_TIMES = 1,  // z = x*y
+    GM_MULT_PLUS = 2,   // z = x+y
+    GM_MULT_PAIR = 3,   // z = 1 (values ignored; "ONEB")
+    GM_MULT_K = 4       // z = k, the inner index (SECONDI on A*B, FIRSTJ after
+                        // the reference's swap; values ignored)
+} gm_mult_op;
+
+typedef struct
+{
+    int32_t add_op;     // gm_add_op
+    int32_t mult_op;    // gm_mult_op
+    int32_t ztype;      // gm_type of the result (and of x,y for valued ops)
+} gm_semiring;
+
+// read-only CSR operand in device memory; rows sorted, no duplicates
+typedef struct
+{
+    int64_t nrows, ncols;
+    const int64_t *p;   // nrows+1 row pointers
+    const int64_t *j;   // p[nrows] column indices
+    const void *x;      // values (NULL: pattern only / all true)
+    int32_t iso;        // 1: x holds one value used for every entry
+    int32_t type;       // gm_type of x
+} gm_csr;
+
+// CSR result; p/j/x are allocated by the library with gm_malloc and owned by
+// the caller afterwards (release with gm_free).  Rows are sorted.
+typedef struct
+{
+    int64_t nrows, ncols;
+    int64_t nvals;      // entries in the result
+    int64_t *p;         // nrows+1
+    int64_t *j;         // capacity >= nvals
+    void *x;            // values (1 value if iso)
+    int32_t iso;
+    int32_t type;
+} gm_csr_out;
+
+// dot3 tuning knobs (0 = default).  Work per mask entry is
+// min(|A(i,:)|+|B(j,:)|, s*ceil(log2(l+1))) with s,l the short/long list.
+typedef struct
+{
+    int64_t thread_max;   // work <= thread_max: one thread per entry (32)
+    int64_t team16_max;   // work <= team16_max: 16 lanes per entry (512)
+    int64_t wave_max;     // work <= wave_max: one wavefront per entry (4096)
+                          // larger: one 256-thread workgroup per entry
+} gm_dot3_params;
+
+// saxpy3 tuning knobs (0 = default).  Row work w = flops + nnz(M(i,:)).
+typedef struct
+{
+    int64_t wave_small_max;   // w <= : one wavefront, 128-slot LDS table (64)
+    int64_t wave_large_max;   // w <= : one wavefront, 1024-slot LDS table (512)
+    int64_t block_max;        // w <= : one workgroup, 4096-slot LDS table (2048)
+                              // larger: workgroup per row, global hash table
+} gm_saxpy3_params;
+
+//------------------------------------------------------------------------------
+// runtime
+//------------------------------------------------------------------------------
+
+int gm_init(void);                  // idempotent; selects the current device
+void gm_finalize(void);             // releases the arena and the stream
+const char *gm_last_error(void);    // text of the last failure on this thread
+int gm_device_count(void);
+
+typedef struct
+{
+    char name[64];
+    char arch[32];          // e.g. "gfx950"
+    int32_t cu_count;
+    int32_t lds_per_block;
+    int64_t hbm_bytes;
+    double hbm_peak_gbs;    // from memory clock and bus width
+} gm_device_info;
+int gm_get_device_info(gm_device_info *info);
+
+void *gm_malloc(size_t bytes);      // device memory from the caching arena
+void gm_free(void *p);              // return to the arena (NULL ok)
+int gm_arena_trim(void);            // give cached blocks back to the driver
+int64_t gm_arena_bytes_in_use(void);
+// host-forced allocation failure (tests): the n-th following gm_malloc
+// fails; n < 0 disables.  Mirrors the malloc_debug count-down of
+// Source/global/GB_Global.c:818-850.
+void gm_debug_fail_after(int64_t n);
+
+int gm_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int gm_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int gm_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int gm_memset(void *dst, int value, size_t bytes);
+int gm_synchronize(void);
+
+// per-kernel timing with HIP events on the library stream
+void gm_prof_enable(int on);
+void gm_prof_reset(void);
+// fills up to cap records; returns the number of distinct kernels timed
+typedef struct { char name[48]; int64_t launches; double total_ms; } gm_prof_record;
+int gm_prof_get(gm_prof_record *rec, int cap);
+
+//------------------------------------------------------------------------------
+// dot3: C<M> = A*B' with C taking the pattern of M (then zombies removed)
+//------------------------------------------------------------------------------
+
+// In CSR terms: for every entry (i,j) of M whose mask value allows it,
+// C(i,j) = (+)_k A(i,k) (x) B(j,k) over the k present in both rows.
+bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
+                    const gm_csr *B, const gm_semiring *sr);
+int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+            const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params);
+
+//------------------------------------------------------------------------------
+// saxpy3: C = A*B, C<M> = A*B, C<!M> = A*B (row-wise Gustavson / hash)
+//------------------------------------------------------------------------------
+
+bool gm_saxpy3_branch(const gm_csr *M, bool mask_comp, bool mask_struct,
+                      const gm_csr *A, const gm_csr *B, const gm_semiring *sr);
+int gm_saxpy3(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct,
+              const gm_csr *A, const gm_csr *B, const gm_semiring *sr,
+              const gm_saxpy3_params *params);
+
+//------------------------------------------------------------------------------
+// reduce a value array to a scalar with a monoid (device -> host scalar)
+//------------------------------------------------------------------------------
+
+// result: host pointer to one value of `type`; x: device array of n values
+// (iso: x[0] repeated n times).
+int gm_reduce_scalar(void *result, int32_t add_op, int32_t type, const void *x,
+                     int64_t n, int32_t iso);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+"""Python wrapper of the CPU oracle (oracle/gm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker / CPU baseline.  The product
+never imports it.
+
+The oracle restates the reference's dot3 (Source/mxm/template/
+GB_AxB_dot3_template.c:111-165 + GB_AxB_dot_cij.c:516-652) and saxpy3
+(coarse-task accumulation of GB_AxB_saxpy3_coarseHash_phase5.c:22-93)
+methods.  Function headers in gm_oracle.c / gm_oracle_template.c cite the
+reference lines each part follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+# semiring name -> (code, ztype numpy dtype, value-type of A/B or None if pattern)
+SEMIRINGS = {
+    "plus_pair_int64": (0, np.int64, None),
+    "plus_pair_int32": (1, np.int32, None),
+    "any_pair_bool": (2, np.uint8, None),
+    "plus_times_fp64": (3, np.float64, np.float64),
+    "min_plus_int32": (4, np.int32, np.int32),
+    "any_secondi_int64": (5, np.int64, None),
+    "plus_times_int64": (6, np.int64, np.int64),
+    "min_plus_int64": (7, np.int64, np.int64),
+    "max_plus_int32": (8, np.int32, np.int32),
+    "min_plus_fp64": (9, np.float64, np.float64),
+}
+
+
+class _OrCsr(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64),
+                ("p", ctypes.c_void_p), ("j", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("iso", ctypes.c_int32), ("xsize", ctypes.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        P = ctypes.c_void_p
+        L.oracle_dot3.restype = ctypes.c_int64
+        L.oracle_dot3.argtypes = [ctypes.c_int, P, ctypes.c_int, P, P, P, P, P]
+        L.oracle_saxpy3.restype = ctypes.c_int64
+        L.oracle_saxpy3.argtypes = [ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P,
+                                    ctypes.c_int]
+        L.oracle_sum_i64.restype = ctypes.c_int64
+        L.oracle_sum_i64.argtypes = [P, ctypes.c_int64]
+        L.oracle_neumaier_sum.restype = ctypes.c_double
+        L.oracle_neumaier_sum.argtypes = [P, ctypes.c_int64]
+        L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _csr(m, keep):
+    """Build the C struct; `keep` collects arrays that must outlive the call."""
+    p = np.ascontiguousarray(m.p, dtype=np.int64)
+    j = np.ascontiguousarray(m.j, dtype=np.int64)
+    x = None if m.x is None else np.ascontiguousarray(m.x)
+    keep += [p, j, x]
+    s = _OrCsr(m.nrows, m.ncols, p.ctypes.data, j.ctypes.data,
+               None if x is None else x.ctypes.data, 1 if m.iso else 0,
+               0 if x is None else x.dtype.itemsize)
+    keep.append(s)
+    return ctypes.byref(s)
+
+
+def dot3(semiring: str, M, A, B, mask_struct=True):
+    """C<M> = A*B' (CSR rows of A and B); returns (Cp, Ci, Cx) compact CSR."""
+    from graphblas_amd.workloads import Csr
+    code, zt, _ = SEMIRINGS[semiring]
+    keep = []
+    mnz = M.nnz
+    Cp = np.empty(M.nrows + 1, dtype=np.int64)
+    Ci = np.empty(max(mnz, 1), dtype=np.int64)
+    Cx = np.empty(max(mnz, 1), dtype=zt)
+    nz = lib().oracle_dot3(code, _csr(M, keep), 1 if mask_struct else 0, _csr(A, keep),
+                           _csr(B, keep), _ptr(Cp), _ptr(Ci), _ptr(Cx))
+    if nz < 0:
+        raise RuntimeError(f"oracle_dot3 failed ({nz})")
+    return Csr(M.nrows, M.ncols, Cp, Ci[:nz].copy(), Cx[:nz].copy())
+
+
+def saxpy3(semiring: str, A, B, M=None, mask_comp=False, mask_struct=True):
+    """C<M>/C<!M>/C = A*B by saxpy3; returns compact CSR with sorted rows."""
+    from graphblas_amd.workloads import Csr
+    code, zt, _ = SEMIRINGS[semiring]
+    keep = []
+    m = A.nrows
+    Cp = np.zeros(m + 1, dtype=np.int64)
+    Mref = None if M is None else _csr(M, keep)
+    Aref, Bref = _csr(A, keep), _csr(B, keep)
+    tot = lib().oracle_saxpy3(code, Mref, int(mask_comp), int(mask_struct), Aref, Bref,
+                              _ptr(Cp), None, None, 0)
+    if tot < 0:
+        raise RuntimeError(f"oracle_saxpy3 failed ({tot})")
+    counts = Cp[:m].copy()
+    Cp[0] = 0
+    np.cumsum(counts, out=Cp[1:])
+    Ci = np.empty(max(tot, 1), dtype=np.int64)
+    Cx = np.empty(max(tot, 1), dtype=zt)
+    lib().oracle_saxpy3(code, Mref, int(mask_comp), int(mask_struct), Aref, Bref,
+                        _ptr(Cp), _ptr(Ci), _ptr(Cx), 1)
+    return Csr(m, B.ncols, Cp, Ci[:tot].copy(), Cx[:tot].copy())
+
+
+def neumaier_sum(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().oracle_neumaier_sum(_ptr(x), x.size)
+
+
+def set_threads(n: int):
+    lib().oracle_set_threads(n)
+
+
+def num_threads() -> int:
+    return lib().oracle_num_threads()

@@ -1,0 +1,122 @@
+"""dot3 (C<M> = A*B') on the GPU against the oracle.
+
+Bar: bit-exact pattern and values for bool/integer semirings; plus_times fp64
+within 1e-12 relative error per entry (summation order may differ).  Calls go
+through the C-ABI (gm_dot3) and through the drop-in GrB_mxm.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from graphblas_amd import capi
+from graphblas_amd import workloads as W
+from helpers import SR, assert_csr_equal, gm_dot3, power_law_csr, rand_csr
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-12   # plus_times fp64, relative, per entry
+
+
+def _vals(handle):
+    return {"GrB_PLUS_TIMES_SEMIRING_FP64": "fp64", "GrB_MIN_PLUS_SEMIRING_INT32": "int32",
+            "GrB_PLUS_TIMES_SEMIRING_INT64": "int64", "GrB_MIN_PLUS_SEMIRING_INT64": "int64",
+            "GrB_MAX_PLUS_SEMIRING_INT32": "int32", "GrB_MIN_PLUS_SEMIRING_FP64": "fp64"}.get(handle)
+
+
+@pytest.mark.parametrize("scale", [8, 10, 12, 14])
+def test_tricount_rmat_grb_mxm(grb, scale):
+    """C<L> = L*U' with plus_pair_int64 (GrB_DESC_ST1), the triangle count."""
+    print(f"[dot3] tricount rmat scale={scale}", flush=True)
+    A, L, U = W.tricount_inputs(scale, 16, seed=1)
+    Ld, Ud = capi.Matrix.from_csr(L, "bool"), capi.Matrix.from_csr(U, "bool")
+    C = capi.Matrix("int64", L.nrows, L.nrows)
+    capi.mxm_checked(C, Ld, "GxB_PLUS_PAIR_INT64", Ld, Ud, "ST1")
+    ref = O.dot3("plus_pair_int64", L, L, U)
+    assert_csr_equal(C.to_csr(), ref, what="tricount")
+    assert capi.reduce_sum_int64(C) == int(ref.x.sum())
+
+
+@pytest.mark.parametrize("handle", list(SR))
+@pytest.mark.parametrize("shape", ["uniform", "powerlaw", "empty_rows"])
+def test_dot3_semirings_cabi(grb, handle, shape):
+    print(f"[dot3] {handle} {shape}", flush=True)
+    rng = np.random.default_rng(hash((handle, shape)) % (2 ** 32))
+    vt = _vals(handle)
+    if shape == "uniform":
+        A = rand_csr(300, 200, 0.05, rng, vt)
+        B = rand_csr(250, 200, 0.05, rng, vt)
+        M = rand_csr(300, 250, 0.05, rng, "int64")
+    elif shape == "powerlaw":
+        A = power_law_csr(400, 12, rng, values=vt)
+        B = power_law_csr(400, 12, rng, values=vt)
+        M = power_law_csr(400, 10, rng, values="int64")
+    else:
+        A = rand_csr(300, 300, 0.03, rng, vt, empty_rows=0.3)
+        B = rand_csr(300, 300, 0.03, rng, vt, empty_rows=0.3)
+        M = rand_csr(300, 300, 0.05, rng, "int64", empty_rows=0.2)
+    name, _ = SR[handle]
+    ref = O.dot3(name, M, A, B, mask_struct=True)
+    got = gm_dot3(handle, M, A, B)
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=handle)
+
+
+@pytest.mark.parametrize("params", [(1, 1, 1), (1, 2, 100000), (1, 10 ** 9, 10 ** 9),
+                                    (10 ** 9, 10 ** 9, 10 ** 9), (0, 0, 0)])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_ANY_SECONDI_INT64"])
+def test_dot3_forced_bins(grb, params, handle):
+    """Every bin (thread / team16 / wave / workgroup) gives the same result."""
+    print(f"[dot3] forced bins {params} {handle}", flush=True)
+    rng = np.random.default_rng(7)
+    vt = _vals(handle)
+    A = power_law_csr(500, 20, rng, values=vt)
+    B = power_law_csr(500, 20, rng, values=vt)
+    M = power_law_csr(500, 15, rng, values="int64")
+    name, _ = SR[handle]
+    ref = O.dot3(name, M, A, B)
+    got = gm_dot3(handle, M, A, B, params=params)
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=str(params))
+
+
+def test_dot3_valued_mask(grb):
+    """A valued mask: entries whose value is 0 do not produce C(i,j)."""
+    rng = np.random.default_rng(3)
+    A = rand_csr(200, 150, 0.08, rng, "int32")
+    B = rand_csr(180, 150, 0.08, rng, "int32")
+    M = rand_csr(200, 180, 0.1, rng, "int64")
+    M.x[rng.random(M.nnz) < 0.4] = 0
+    ref = O.dot3("min_plus_int32", M, A, B, mask_struct=False)
+    got = gm_dot3("GrB_MIN_PLUS_SEMIRING_INT32", M, A, B, mask_struct=False)
+    assert_csr_equal(got, ref, what="valued mask")
+
+
+def test_dot3_empty(grb):
+    rng = np.random.default_rng(4)
+    A = rand_csr(50, 40, 0.0, rng)
+    B = rand_csr(60, 40, 0.1, rng)
+    M = rand_csr(50, 60, 0.0, rng, "int64")
+    got = gm_dot3("GxB_PLUS_PAIR_INT64", M, A, B)
+    assert got.nnz == 0 and np.all(got.p == 0)
+    M2 = rand_csr(50, 60, 0.2, rng, "int64")
+    got = gm_dot3("GxB_PLUS_PAIR_INT64", M2, A, B)
+    assert got.nnz == 0 and np.all(got.p == 0)
+
+
+def test_dot3_grb_mxm_min_plus_batch_shape(grb):
+    """C<A> = A*A' with min_plus int32 on a weighted RMAT (the batched shape)."""
+    G = W.rmat_sym(11, 8, seed=5)
+    G.x = W.sym_weights_i32(G, seed=9)
+    Gd = capi.Matrix.from_csr(G, "int32")
+    C = capi.Matrix("int32", G.nrows, G.nrows)
+    capi.mxm_checked(C, Gd, "GrB_MIN_PLUS_SEMIRING_INT32", Gd, Gd, "ST1")
+    ref = O.dot3("min_plus_int32", G, G, G)
+    assert_csr_equal(C.to_csr(), ref, what="min_plus C<A>=A*A'")
+
+
+def test_dot3_deterministic_fp64(grb):
+    rng = np.random.default_rng(12)
+    A = power_law_csr(400, 30, rng, values="fp64")
+    M = power_law_csr(400, 30, rng, values="int64")
+    r1 = gm_dot3("GrB_PLUS_TIMES_SEMIRING_FP64", M, A, A)
+    r2 = gm_dot3("GrB_PLUS_TIMES_SEMIRING_FP64", M, A, A)
+    assert np.array_equal(r1.x, r2.x)

@@ -1,0 +1,147 @@
+"""saxpy3 (C = A*B, C<M> = A*B, C<!M> = A*B) on the GPU against the oracle.
+
+Bar: bit-exact for bool/integer semirings and ANY (the GPU keeps the last
+writer in the reference's k order, like the oracle); plus_times fp64 within
+1e-12 relative error per entry (the numeric phase applies products in the
+reference's order, so it is usually bit-exact too).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from graphblas_amd import capi
+from graphblas_amd import workloads as W
+from helpers import SR, assert_csr_equal, gm_saxpy3, power_law_csr, rand_csr
+
+pytestmark = pytest.mark.gpu
+
+FP_TOL = 1e-12
+
+
+def _vals(handle):
+    return {"GrB_PLUS_TIMES_SEMIRING_FP64": "fp64", "GrB_MIN_PLUS_SEMIRING_INT32": "int32",
+            "GrB_PLUS_TIMES_SEMIRING_INT64": "int64", "GrB_MIN_PLUS_SEMIRING_INT64": "int64",
+            "GrB_MAX_PLUS_SEMIRING_INT32": "int32", "GrB_MIN_PLUS_SEMIRING_FP64": "fp64"}.get(handle)
+
+
+@pytest.mark.parametrize("n", [1024, 1 << 14])
+def test_er_plus_times_grb_mxm(grb, n):
+    """C = A*B, plus_times fp64, ER with exactly 8 entries per row (GrB_mxm)."""
+    print(f"[saxpy3] ER n={n}", flush=True)
+    A, B = W.er(n, n, 8, seed=1), W.er(n, n, 8, seed=2)
+    Ad, Bd = capi.Matrix.from_csr(A, "fp64"), capi.Matrix.from_csr(B, "fp64")
+    C = capi.Matrix("fp64", n, n)
+    capi.mxm_checked(C, None, "GrB_PLUS_TIMES_SEMIRING_FP64", Ad, Bd)
+    ref = O.saxpy3("plus_times_fp64", A, B)
+    got = C.to_csr()
+    assert_csr_equal(got, ref, fp_tol=FP_TOL, what="ER")
+    # ascending-k accumulation order: identical bits to the oracle
+    assert np.array_equal(got.x, ref.x)
+
+
+@pytest.mark.parametrize("handle", list(SR))
+@pytest.mark.parametrize("shape", ["uniform", "powerlaw", "empty_rows"])
+def test_saxpy3_semirings_cabi(grb, handle, shape):
+    print(f"[saxpy3] {handle} {shape}", flush=True)
+    rng = np.random.default_rng(hash((handle, shape, 1)) % (2 ** 32))
+    vt = _vals(handle)
+    if shape == "uniform":
+        A = rand_csr(300, 200, 0.04, rng, vt)
+        B = rand_csr(200, 250, 0.04, rng, vt)
+    elif shape == "powerlaw":
+        A = power_law_csr(500, 10, rng, values=vt)
+        B = power_law_csr(500, 10, rng, values=vt)
+    else:
+        A = rand_csr(300, 300, 0.03, rng, vt, empty_rows=0.3)
+        B = rand_csr(300, 300, 0.03, rng, vt, empty_rows=0.3)
+    name, _ = SR[handle]
+    ref = O.saxpy3(name, A, B)
+    got = gm_saxpy3(handle, A, B)
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=handle)
+
+
+@pytest.mark.parametrize("comp", [False, True])
+@pytest.mark.parametrize("struct", [True, False])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GxB_ANY_SECONDI_INT64", "GrB_MIN_PLUS_SEMIRING_INT32"])
+def test_saxpy3_masked(grb, comp, struct, handle):
+    print(f"[saxpy3] masked comp={comp} struct={struct} {handle}", flush=True)
+    rng = np.random.default_rng(21 + comp + 2 * struct)
+    vt = _vals(handle)
+    A = power_law_csr(400, 8, rng, values=vt)
+    B = power_law_csr(400, 8, rng, values=vt)
+    M = rand_csr(400, 400, 0.05, rng, "int64")
+    if not struct:
+        M.x[rng.random(M.nnz) < 0.3] = 0
+    name, _ = SR[handle]
+    ref = O.saxpy3(name, A, B, M=M, mask_comp=comp, mask_struct=struct)
+    got = gm_saxpy3(handle, A, B, M=M, mask_comp=comp, mask_struct=struct)
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what="masked")
+
+
+@pytest.mark.parametrize("params", [(1, 1, 1), (64, 512, 2048), (1, 512, 2048), (64, 64, 64)])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GxB_ANY_SECONDI_INT64"])
+def test_saxpy3_forced_bins(grb, params, handle):
+    """wave128 / wave1024 / block4096 / global tables agree."""
+    print(f"[saxpy3] forced bins {params} {handle}", flush=True)
+    rng = np.random.default_rng(5)
+    vt = _vals(handle)
+    A = power_law_csr(600, 12, rng, values=vt)
+    B = power_law_csr(600, 12, rng, values=vt)
+    name, _ = SR[handle]
+    ref = O.saxpy3(name, A, B)
+    got = gm_saxpy3(handle, A, B, params=params)
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=str(params))
+
+
+def test_bfs_step_grb_mxm(grb):
+    """C<!V> = F*A with any_secondi int64 and GrB_DESC_RSC (frontier expansion)."""
+    G = W.rmat_sym(12, 16, seed=3)
+    n = G.nrows
+    rng = np.random.default_rng(8)
+    k = 37
+    src = rng.choice(n, size=k, replace=False)
+    F = W.Csr(k, n, np.arange(k + 1, dtype=np.int64), np.sort(src)[np.argsort(np.argsort(src))].astype(np.int64))
+    F = W.Csr(k, n, np.arange(k + 1, dtype=np.int64), src.astype(np.int64))
+    V = F  # visited = the sources
+    Fd = capi.Matrix.from_csr(F, "bool")
+    Gd = capi.Matrix.from_csr(G, "bool")
+    Vd = capi.Matrix.from_csr(V, "bool")
+    C = capi.Matrix("int64", k, n)
+    capi.mxm_checked(C, Vd, "GxB_ANY_SECONDI_INT64", Fd, Gd, "RSC")
+    got = C.to_csr()
+    ref = O.saxpy3("any_secondi_int64", F, G, M=V, mask_comp=True, mask_struct=True)
+    assert_csr_equal(got, ref, what="bfs")
+    # witness validity: C(r, i) = k requires F(r, k) and A(k, i)
+    for r in range(k):
+        for q in range(got.p[r], got.p[r + 1]):
+            kk = got.x[q]
+            assert kk == src[r]
+            assert got.j[q] in G.j[G.p[kk]:G.p[kk + 1]]
+
+
+def test_saxpy3_rmat_square_heavy_rows(grb):
+    """A*A on a power-law graph: heavy rows exercise block and global tables."""
+    G = W.rmat_sym(10, 16, seed=2)
+    Gd = capi.Matrix.from_csr(G, "bool")
+    C = capi.Matrix("int64", G.nrows, G.nrows)
+    capi.mxm_checked(C, None, "GxB_PLUS_PAIR_INT64", Gd, Gd)
+    ref = O.saxpy3("plus_pair_int64", G, G)
+    assert_csr_equal(C.to_csr(), ref, what="A*A")
+
+
+def test_saxpy3_empty(grb):
+    rng = np.random.default_rng(1)
+    A = rand_csr(40, 30, 0.0, rng, "fp64")
+    B = rand_csr(30, 50, 0.2, rng, "fp64")
+    got = gm_saxpy3("GrB_PLUS_TIMES_SEMIRING_FP64", A, B)
+    assert got.nnz == 0 and np.all(got.p == 0)
+
+
+def test_saxpy3_deterministic(grb):
+    rng = np.random.default_rng(2)
+    A = power_law_csr(800, 20, rng, values="fp64")
+    r1 = gm_saxpy3("GrB_PLUS_TIMES_SEMIRING_FP64", A, A)
+    r2 = gm_saxpy3("GrB_PLUS_TIMES_SEMIRING_FP64", A, A)
+    assert np.array_equal(r1.x, r2.x) and np.array_equal(r1.j, r2.j)

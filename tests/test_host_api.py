@@ -1,0 +1,201 @@
+"""CPU-only checks of the drop-in boundary: the library loads and exports every
+function the public headers declare, global handles exist, and the host
+logic (descriptor decode, semiring decode, method selection, argument
+validation, branch predicates) behaves like the reference's.  No call here
+touches a GPU.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from graphblas_amd import capi
+from graphblas_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("GraphBLAS.h", "gm_kernels.h")]
+
+
+def declared_functions(path):
+    src = open(path).read()
+    src = re.sub(r"//.*", "", src)
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_\s\*]*?\b([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                         src, flags=re.M):
+        name = m.group(1)
+        if name in ("if", "for", "while", "switch", "return", "sizeof", "defined"):
+            continue
+        line = src[m.start():src.find("\n", m.start())]
+        if line.lstrip().startswith("#"):
+            continue
+        names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_declared_functions(header):
+    L = capi.lib()
+    names = declared_functions(header)
+    assert len(names) > 10
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_global_handles_exist():
+    for name in list(capi.SEMIRINGS) + ["GrB_DESC_" + d for d in capi.DESCRIPTORS] + \
+            list(capi.GRB_TYPE_NAMES.values()) + ["GrB_PLUS_MONOID_INT64", "GrB_LOR",
+                                                  "GxB_SECONDI_INT64"]:
+        assert capi.handle(name), name
+
+
+def test_semiring_decode():
+    L = capi.lib()
+    cases = {
+        "GxB_PLUS_PAIR_INT64": (1, 3, 8), "GxB_PLUS_PAIR_INT32": (1, 3, 6),
+        "GxB_ANY_PAIR_BOOL": (4, 3, 1), "GxB_LOR_PAIR_BOOL": (4, 3, 1),  # renamed to ANY_PAIR
+        "GrB_PLUS_TIMES_SEMIRING_FP64": (1, 1, 11), "GrB_MIN_PLUS_SEMIRING_INT32": (2, 2, 6),
+        "GxB_ANY_SECONDI_INT64": (4, 4, 8), "GxB_ANY_FIRSTJ_INT64": (4, 4, 8),
+    }
+    for name, trip in cases.items():
+        s = capi.GmSemiring()
+        assert L.GB_semiring_codes(capi.handle(name), ctypes.byref(s))
+        assert (s.add_op, s.mult_op, s.ztype) == trip, name
+
+
+def test_method_selection():
+    sel = capi.lib().GB_mxm_select
+    # (mask, comp, a_tran, b_tran) -> 1 dot3, 2 saxpy3, 0 not handled
+    assert sel(True, False, False, True, 0) == 1       # C<M>=A*B'  (tricount, ST1)
+    assert sel(True, True, False, True, 0) == 0        # C<!M>=A*B' needs dot2
+    assert sel(False, False, False, True, 0) == 0      # C=A*B' needs a transpose
+    assert sel(False, False, False, False, 0) == 2     # C=A*B
+    assert sel(True, False, False, False, 0) == 2      # C<M>=A*B
+    assert sel(True, True, False, False, 0) == 2       # C<!M>=A*B (BFS, RSC)
+    assert sel(True, False, True, False, 0) == 0       # A' operand
+
+
+def test_branch_predicates_host_only():
+    L = capi.lib()
+    s = capi.GmSemiring(1, 3, 8)
+    A = capi.GmCsr(10, 5, 1, 1, None, 0, 1)
+    B = capi.GmCsr(7, 5, 1, 1, None, 0, 1)
+    M = capi.GmCsr(10, 7, 1, 1, None, 0, 8)
+    assert L.gm_dot3_branch(ctypes.byref(M), True, ctypes.byref(A), ctypes.byref(B),
+                            ctypes.byref(s))
+    Mbad = capi.GmCsr(10, 8, 1, 1, None, 0, 8)
+    assert not L.gm_dot3_branch(ctypes.byref(Mbad), True, ctypes.byref(A), ctypes.byref(B),
+                                ctypes.byref(s))
+    bad = capi.GmSemiring(1, 9, 8)
+    assert not L.gm_dot3_branch(ctypes.byref(M), True, ctypes.byref(A), ctypes.byref(B),
+                                ctypes.byref(bad))
+    # valued semiring needs values of the semiring's type
+    pt = capi.GmSemiring(1, 1, 11)
+    assert not L.gm_dot3_branch(ctypes.byref(M), True, ctypes.byref(A), ctypes.byref(B),
+                                ctypes.byref(pt))
+    A2 = capi.GmCsr(5, 7, 1, 1, None, 0, 1)
+    B2 = capi.GmCsr(7, 9, 1, 1, None, 0, 1)
+    assert L.gm_saxpy3_branch(None, False, True, ctypes.byref(A2), ctypes.byref(B2),
+                              ctypes.byref(s))
+    assert not L.gm_saxpy3_branch(None, False, True, ctypes.byref(B2), ctypes.byref(A2),
+                                  ctypes.byref(s))
+
+
+def _mat(tname, nr, nc):
+    return capi.Matrix(tname, nr, nc)
+
+
+def test_grb_mxm_argument_errors_without_gpu():
+    capi.init()
+    L = capi.lib()
+    A = _mat("int64", 4, 5)
+    B = _mat("int64", 6, 5)
+    C = _mat("int64", 4, 6)
+    M = _mat("int64", 4, 6)
+    sr = "GxB_PLUS_PAIR_INT64"
+    # dimension mismatch: A*B needs ncols(A) == nrows(B)
+    assert capi.mxm(C, None, sr, A, B) == capi.GrB_DIMENSION_MISMATCH
+    # NULL semiring
+    assert L.GrB_mxm(C.h, None, None, None, A.h, B.h, None) == capi.GrB_NULL_POINTER
+    # NULL output
+    assert L.GrB_mxm(None, None, None, capi.handle(sr), A.h, B.h, None) == capi.GrB_NULL_POINTER
+    # result type must be C's type
+    Cf = _mat("fp64", 4, 6)
+    assert capi.mxm(Cf, M, sr, A, B, "ST1") == capi.GrB_NOT_IMPLEMENTED
+    # accum is outside this library
+    acc = capi.handle("GrB_PLUS_INT64")
+    assert L.GrB_mxm(C.h, M.h, acc, capi.handle(sr), A.h, B.h,
+                     capi.desc("ST1")) == capi.GrB_NOT_IMPLEMENTED
+    # A' operand
+    At = _mat("int64", 5, 4)
+    assert capi.mxm(C, M, sr, At, B, "ST0T1") == capi.GrB_NOT_IMPLEMENTED
+    # mask dimension
+    M2 = _mat("int64", 4, 7)
+    assert capi.mxm(C, M2, sr, A, B, "ST1") == capi.GrB_DIMENSION_MISMATCH
+    # the error string is attached to C
+    s = ctypes.c_char_p()
+    L.GrB_Matrix_error(ctypes.byref(s), C.h)
+    assert b"dimension" in s.value.lower() or b"mask" in s.value.lower()
+
+
+def test_descriptor_set_semantics():
+    L = capi.lib()
+    d = ctypes.c_void_p()
+    assert L.GrB_Descriptor_new(ctypes.byref(d)) == 0
+    assert L.GrB_Descriptor_set(d, 1, 2) == 0       # MASK, COMP
+    assert L.GrB_Descriptor_set(d, 1, 4) == 0       # MASK, STRUCTURE (accumulates)
+    assert L.GrB_Descriptor_set(d, 3, 3) == 0       # INP1, TRAN
+    assert L.GrB_Descriptor_set(d, 0, 3) == capi.GrB_INVALID_VALUE
+    # predefined descriptors are read-only
+    assert L.GrB_Descriptor_set(capi.desc("ST1"), 0, 1) == capi.GrB_INVALID_VALUE
+    assert L.GrB_Descriptor_free(ctypes.byref(d)) == 0
+    assert d.value is None
+
+
+def test_semiring_new_domain_check():
+    L = capi.lib()
+    s = ctypes.c_void_p()
+    rc = L.GrB_Semiring_new(ctypes.byref(s), capi.handle("GrB_PLUS_MONOID_FP64"),
+                            capi.handle("GrB_TIMES_INT64"))
+    assert rc == capi.GrB_DOMAIN_MISMATCH
+    rc = L.GrB_Semiring_new(ctypes.byref(s), capi.handle("GrB_MIN_MONOID_INT64"),
+                            capi.handle("GrB_PLUS_INT64"))
+    assert rc == 0
+    c = capi.GmSemiring()
+    assert L.GB_semiring_codes(s, ctypes.byref(c)) and (c.add_op, c.mult_op, c.ztype) == (2, 2, 8)
+    assert L.GrB_Semiring_free(ctypes.byref(s)) == 0
+
+
+def test_generators_are_well_formed():
+    A = W.rmat_sym(9, 8, seed=4)
+    n = A.nrows
+    assert A.p[0] == 0 and np.all(np.diff(A.p) >= 0)
+    for i in range(n):
+        row = A.j[A.p[i]:A.p[i + 1]]
+        assert np.all(np.diff(row) > 0)         # sorted, no duplicates
+        assert not np.any(row == i)              # no self-loops
+    S = A.to_scipy()
+    assert (S != S.T).nnz == 0                   # symmetric
+    E = W.er(1000, 777, 8, seed=3)
+    assert np.all(np.diff(E.p) == 8)
+    for i in range(0, 1000, 37):
+        row = E.j[E.p[i]:E.p[i + 1]]
+        assert len(set(row.tolist())) == 8 and row.max() < 777 and np.all(np.diff(row) > 0)
+    assert np.all((E.x >= 0) & (E.x < 1))
+    # deterministic
+    A2 = W.rmat_sym(9, 8, seed=4)
+    assert np.array_equal(A.p, A2.p) and np.array_equal(A.j, A2.j)
+
+
+def test_no_gpu_in_container_reports_cleanly():
+    """Without a device, a compute call fails loudly (GrB_PANIC), never falls back."""
+    L = capi.lib()
+    if L.gm_device_count() > 0:
+        pytest.skip("a GPU is present")
+    capi.init()
+    A = _mat("int64", 4, 4)
+    C = _mat("int64", 4, 4)
+    rc = capi.mxm(C, None, "GxB_PLUS_PAIR_INT64", A, A)
+    assert rc == capi.GrB_PANIC
