@@ -95,7 +95,7 @@ class GmDot3Params(ctypes.Structure):
 
 class GmSaxpy3Params(ctypes.Structure):
     _fields_ = [("wave_small_max", ctypes.c_int64), ("wave_large_max", ctypes.c_int64),
-                ("block_max", ctypes.c_int64)]
+                ("block_max", ctypes.c_int64), ("mode", ctypes.c_int64)]
 
 
 class GmProfRecord(ctypes.Structure):

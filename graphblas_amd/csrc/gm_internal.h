@@ -19,6 +19,7 @@ int ensure_init();                          // gm_init() result, cached
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 int cu_count();
+int64_t device_hbm_bytes();                // total HBM of the current device
 
 // timed launch bracket: when profiling is on, events are recorded around the
 // launch on the library stream and attributed to `name`

@@ -27,6 +27,7 @@ static int g_init_rc = 1;             // 1: not yet initialised
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
 static int g_cu = 0;
+static int64_t g_hbm = 0;
 static thread_local char g_err[512] = "";
 
 int set_error(int code, const char *fmt, ...)
@@ -48,6 +49,7 @@ int check_hip(hipError_t e, const char *what)
 
 hipStream_t stream() { return g_stream; }
 int cu_count() { return g_cu; }
+int64_t device_hbm_bytes() { return g_hbm; }
 
 int ensure_init()
 {
@@ -295,6 +297,7 @@ int gm_init(void)
         return g_init_rc;
     }
     g_cu = prop.multiProcessorCount;
+    g_hbm = (int64_t) prop.totalGlobalMem;
     if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
     {
         g_init_rc = set_error(GM_PANIC, "gm_init: stream creation failed");

@@ -143,46 +143,128 @@ __device__ __forceinline__ int32_t sx_insert(uint32_t *keys, int logT, uint32_t 
 __device__ __forceinline__ int32_t ilog2_floor32(uint32_t x) { return 31 - __clz(x); }
 
 //------------------------------------------------------------------------------
-// flops + binning
+// team-of-TG helpers (TG lanes of one wavefront work on one row)
 //------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g)
+constexpr int TG = 8;          // lanes per row in the flop-count and fused kernels
+constexpr int FMAX = 64;       // largest flop count of a "small" row
+
+template <class T>
+__device__ __forceinline__ T team_incl_scan(T v)
 {
-    for (int64_t base = (int64_t) blockIdx.x * SX_NT; base < g.m; base += (int64_t) gridDim.x * SX_NT)
+    const int sub = threadIdx.x & (TG - 1);
+    #pragma unroll
+    for (int off = 1; off < TG; off <<= 1)
     {
-        const int64_t i = base + threadIdx.x;
-        int bin = -1;
-        int64_t t = 0;
+        T u = __shfl_up(v, off, TG);
+        if (sub >= off) v += u;
+    }
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T team_sum(T v)
+{
+    #pragma unroll
+    for (int off = 1; off < TG; off <<= 1) v += __shfl_xor(v, off, TG);
+    return v;
+}
+
+//------------------------------------------------------------------------------
+// max row lengths of A and B (decides whether every row is "small")
+//------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__ Ap, int64_t m,
+                                                     const int64_t *__restrict__ Bp, int64_t kb,
+                                                     unsigned long long *out)
+{
+    __shared__ unsigned long long s[2][SX_NT / 64];
+    unsigned long long ma = 0, mb = 0;
+    const int64_t lim = m > kb ? m : kb;
+    for (int64_t i = (int64_t) blockIdx.x * SX_NT + threadIdx.x; i < lim;
+         i += (int64_t) gridDim.x * SX_NT)
+    {
+        if (i < m) { unsigned long long l = (unsigned long long) (Ap[i + 1] - Ap[i]); ma = l > ma ? l : ma; }
+        if (i < kb) { unsigned long long l = (unsigned long long) (Bp[i + 1] - Bp[i]); mb = l > mb ? l : mb; }
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        unsigned long long ua = __shfl_xor(ma, off, 64), ub = __shfl_xor(mb, off, 64);
+        ma = ua > ma ? ua : ma;
+        mb = ub > mb ? ub : mb;
+    }
+    if ((threadIdx.x & 63) == 0) { s[0][threadIdx.x >> 6] = ma; s[1][threadIdx.x >> 6] = mb; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < SX_NT / 64; w++)
+        {
+            ma = s[0][w] > ma ? s[0][w] : ma;
+            mb = s[1][w] > mb ? s[1][w] : mb;
+        }
+        atomicMax(&out[0], ma);
+        atomicMax(&out[1], mb);
+    }
+}
+
+//------------------------------------------------------------------------------
+// flops + binning: TG lanes per row (coalesced reads of A(i,:) and of the
+// B row pointers it names).  Rows with f <= fs are "small"; their flops are
+// summed into cnt[7] (capacity for the fused kernel) and they are listed in
+// BIN_WS (used only when the fused kernel is not).  Larger rows go to the
+// single / wave1024 / block4096 / global bins.
+//------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs)
+{
+    const int sub = threadIdx.x & (TG - 1);
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = (int64_t) blockIdx.x * (SX_NT / TG); base < g.m;
+         base += (int64_t) gridDim.x * (SX_NT / TG))
+    {
+        const int64_t i = base + threadIdx.x / TG;
+        int64_t f = 0, a0 = 0, a1 = 0;
         if (i < g.m)
         {
-            const int64_t a0 = g.Ap[i], a1 = g.Ap[i + 1];
-            int64_t f = 0;
-            for (int64_t e = a0; e < a1; e++)
+            a0 = g.Ap[i];
+            a1 = g.Ap[i + 1];
+            for (int64_t e = a0 + sub; e < a1; e += TG)
             {
                 const int64_t k = g.Aj[e];
                 f += g.Bp[k + 1] - g.Bp[k];
             }
-            t = f;
+        }
+        f = team_sum<int64_t>(f);
+        int bin = -1;
+        int64_t t = f;
+        int64_t fsmall = 0;
+        if (i < g.m && sub == 0)
+        {
             if (g.mode == MODE_MASK)
             {
                 const int64_t mi = g.Mp[i + 1] - g.Mp[i];
                 if (mi < t) t = mi;
             }
             if (t > g.n) t = g.n;
-            if (f == 0 || t == 0) { g.Cp[i] = 0; }
+            if (f == 0 || t == 0) g.Cp[i] = 0;
+            else if (f <= fs) { bin = BIN_WS; fsmall = f; }
             else if (a1 - a0 == 1) bin = BIN_SINGLE;
-            else if (t <= g.ws_max) bin = BIN_WS;
             else if (t <= g.wl_max) bin = BIN_WL;
             else if (t <= g.blk_max) bin = BIN_BLK;
             else bin = BIN_GLB;
         }
+        // flops of small rows (upper bound of their output size)
+        unsigned long long wsum = (unsigned long long) fsmall;
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
+        if (lane == 0 && wsum) atomicAdd(&g.cnt[7], wsum);
         #pragma unroll
         for (int b = 0; b < SX_NBINS; b++)
         {
             const bool take = (bin == b);
             const unsigned long long mask = __ballot(take);
             if (mask == 0) continue;
-            const int lane = threadIdx.x & 63;
             const int leader = __ffsll((long long) mask) - 1;
             unsigned long long slot = 0;
             if (lane == leader) slot = atomicAdd(&g.cnt[b], (unsigned long long) __popcll(mask));
@@ -224,7 +306,11 @@ __global__ __launch_bounds__(SX_NT) void k_sx_single(SxArgs g)
         const int64_t pm0 = g.Mp ? g.Mp[i] : 0, pm1 = g.Mp ? g.Mp[i + 1] : 0;
         if (PHASE == 0 && g.mode == MODE_NONE)
         {
-            if (lane == 0) g.Cp[i] = b1 - b0;
+            if (lane == 0)
+            {
+                g.Cp[i] = b1 - b0;
+                atomicAdd(&g.cnt[6], (unsigned long long) (b1 - b0));
+            }
             continue;
         }
         const Z av = S::geta((const Z *) g.Ax, e, g.a_iso);
@@ -254,7 +340,11 @@ __global__ __launch_bounds__(SX_NT) void k_sx_single(SxArgs g)
             }
             run += __popcll(bal);
         }
-        if (PHASE == 0 && lane == 0) g.Cp[i] = run;
+        if (PHASE == 0 && lane == 0)
+        {
+            g.Cp[i] = run;
+            atomicAdd(&g.cnt[6], (unsigned long long) run);
+        }
     }
 }
 
@@ -425,7 +515,11 @@ __device__ void sx_row(const SxArgs &g, int64_t i, uint32_t *keys, typename S::Z
         {
             tot = block_sum<int64_t, NT>(nnew, s_red);
         }
-        if (tid == 0) g.Cp[i] = tot;
+        if (tid == 0)
+        {
+            g.Cp[i] = tot;
+            atomicAdd(&g.cnt[6], (unsigned long long) tot);
+        }
         return;
     }
 
@@ -557,6 +651,266 @@ __global__ void k_sx_iso_one(void *x, int size)
     }
 }
 
+//------------------------------------------------------------------------------
+// single-pass fused kernel for small rows (f <= fs <= 64)
+//------------------------------------------------------------------------------
+//
+// A persistent grid takes tiles of SX_NT/TG = 32 consecutive rows in ticket
+// order.  One team of TG = 8 lanes per row:
+//   expand   the row's flops, in the reference's order (A(i,:) in chunks of
+//            TG entries, B(k,:) in order), into LDS as sort keys
+//            (j << 6 | flop index) plus values; masked-out flops stay at the
+//            all-ones key
+//   sort     bitonic sort of the keys (the flop index makes them unique and
+//            keeps equal columns in flop order)
+//   compress runs of one column are reduced in flop order, so fp64 sums are
+//            the reference's sequential ascending-k sums and ANY keeps the
+//            last writer
+// Rows with f > fs ("big") were counted beforehand by the hash kernels; the
+// tile only adds their counts to the scan.  The tile's output offset comes
+// from a decoupled look-back over the previous tiles' published counts
+// (status word: 2-bit flag | 62-bit count); then C(tile rows) is written with
+// contiguous, coalesced stores, together with Cp.  Ticket order means a tile
+// only waits on tiles already running; a bounded spin turns a lost
+// predecessor into an error flag instead of a hang.
+
+constexpr int FUSED_ROWS = SX_NT / TG;
+constexpr unsigned long long LB_AGG = 1ull << 62;
+constexpr unsigned long long LB_PRE = 2ull << 62;
+constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
+constexpr int LB_SPIN_LIMIT = 1 << 22;
+
+template <class S>
+__global__ __launch_bounds__(SX_NT) void k_sx_fused(SxArgs g, int64_t fs,
+                                                    unsigned long long *status, int64_t ntiles,
+                                                    int *err)
+{
+    using Z = typename S::Z;
+    __shared__ uint64_t s_key[FUSED_ROWS][FMAX];    // sort keys, then output columns (int32)
+    __shared__ Z s_val[FUSED_ROWS][FMAX];           // flop values, then output values
+    __shared__ int64_t s_pre[FUSED_ROWS][TG], s_k[FUSED_ROWS][TG], s_bs[FUSED_ROWS][TG];
+    __shared__ Z s_av[FUSED_ROWS][TG];
+    __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
+    __shared__ int s_big[FUSED_ROWS];
+    __shared__ int64_t s_tile, s_prefix;
+
+    const int team = threadIdx.x / TG, sub = threadIdx.x & (TG - 1);
+    uint64_t *key = s_key[team];
+    Z *val = s_val[team];
+    int32_t *outj = (int32_t *) s_key[team];
+    unsigned long long *ticket = status + ntiles;
+
+    while (true)
+    {
+        if (threadIdx.x == 0) s_tile = (int64_t) atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const int64_t tile = s_tile;
+        if (tile >= ntiles) break;
+        const int64_t i = tile * FUSED_ROWS + team;
+
+        #pragma unroll
+        for (int u = 0; u < FMAX / TG; u++) key[sub * (FMAX / TG) + u] = ~0ull;
+        int64_t f = 0, cnt = 0;
+        int big = 0, nallow = 0;
+        if (i < g.m)
+        {
+            const int64_t a0 = g.Ap[i], a1 = g.Ap[i + 1];
+            const int64_t pm0 = g.Mp ? g.Mp[i] : 0, pm1 = g.Mp ? g.Mp[i + 1] : 0;
+            wave_sync();
+            for (int64_t e0 = a0; e0 < a1; e0 += TG)
+            {
+                const int64_t e = e0 + sub;
+                int64_t k = 0, bs = 0, len = 0;
+                Z av = S::identity();
+                if (e < a1)
+                {
+                    k = g.Aj[e];
+                    bs = g.Bp[k];
+                    len = g.Bp[k + 1] - bs;
+                    av = S::geta((const Z *) g.Ax, e, g.a_iso);
+                }
+                const int64_t inc = team_incl_scan<int64_t>(len);
+                const int64_t tot = __shfl(inc, TG - 1, TG);
+                if (f + tot > fs) { big = 1; break; }
+                s_pre[team][sub] = inc - len;
+                s_k[team][sub] = k;
+                s_bs[team][sub] = bs;
+                s_av[team][sub] = av;
+                const int nA = (a1 - e0) < TG ? (int) (a1 - e0) : TG;
+                wave_sync();
+                #pragma unroll
+                for (int it = 0; it < FMAX / TG; it++)
+                {
+                    const int64_t q = (int64_t) it * TG + sub;
+                    if (q < tot)
+                    {
+                        int lo = 0, hi = nA - 1;
+                        while (lo < hi)
+                        {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_pre[team][mid] <= q) lo = mid; else hi = mid - 1;
+                        }
+                        const int64_t pb = s_bs[team][lo] + (q - s_pre[team][lo]);
+                        const int64_t j = g.Bj[pb];
+                        if (sx_mask_ok(g, pm0, pm1, j))
+                        {
+                            const int64_t fq = f + q;
+                            key[fq] = ((uint64_t) j << 6) | (uint64_t) fq;
+                            if constexpr (!S::iso_out)
+                                val[fq] = S::mul(s_av[team][lo], S::geta((const Z *) g.Bx, pb, g.b_iso),
+                                                 s_k[team][lo]);
+                            nallow++;
+                        }
+                    }
+                }
+                f += tot;
+                wave_sync();
+            }
+            if (big) cnt = g.Cp[i];
+        }
+        nallow = team_sum<int>(nallow);
+        if (!big && nallow > 0)
+        {
+            // bitonic sort of key[0..P)
+            int P = 2;
+            while (P < f) P <<= 1;
+            wave_sync();
+            for (int kk = 2; kk <= P; kk <<= 1)
+            {
+                for (int jj = kk >> 1; jj > 0; jj >>= 1)
+                {
+                    for (int t = sub; t < P / 2; t += TG)
+                    {
+                        const int lo = 2 * t - (t & (jj - 1));
+                        const int hi = lo + jj;
+                        const bool asc = (lo & kk) == 0;
+                        const uint64_t a = key[lo], b = key[hi];
+                        if ((a > b) == asc) { key[lo] = b; key[hi] = a; }
+                    }
+                    wave_sync();
+                }
+            }
+            // compress: lane owns sorted positions sub*8 .. sub*8+7
+            constexpr int PER = FMAX / TG;
+            int32_t hj[PER];
+            Z hz[PER];
+            int nh = 0;
+            #pragma unroll
+            for (int u = 0; u < PER; u++)
+            {
+                const int p = sub * PER + u;
+                hj[u] = -1;
+                if (p < nallow)
+                {
+                    const uint64_t kp = key[p];
+                    const int64_t jp = (int64_t) (kp >> 6);
+                    if (p == 0 || (int64_t) (key[p - 1] >> 6) != jp)
+                    {
+                        Z z = S::identity();
+                        if constexpr (!S::iso_out)
+                        {
+                            z = val[kp & 63];
+                            for (int p2 = p + 1; p2 < nallow; p2++)
+                            {
+                                const uint64_t k2 = key[p2];
+                                if ((int64_t) (k2 >> 6) != jp) break;
+                                z = S::upd(z, val[k2 & 63]);
+                            }
+                        }
+                        hj[u] = (int32_t) jp;
+                        hz[u] = z;
+                        nh++;
+                    }
+                }
+            }
+            const int inc = team_incl_scan<int>(nh);
+            int r = inc - nh;
+            cnt = __shfl(inc, TG - 1, TG);
+            wave_sync();
+            #pragma unroll
+            for (int u = 0; u < PER; u++)
+            {
+                if (hj[u] >= 0)
+                {
+                    outj[r] = hj[u];
+                    if constexpr (!S::iso_out) val[r] = hz[u];
+                    r++;
+                }
+            }
+        }
+        if (sub == 0)
+        {
+            s_cnt[team] = cnt;
+            s_big[team] = big;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            int64_t run = 0;
+            for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = run; run += s_cnt[t]; }
+            s_off[FUSED_ROWS] = run;
+            int64_t prefix = 0;
+            if (tile == 0)
+            {
+                __hip_atomic_store(&status[0], LB_PRE | (unsigned long long) run, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            else
+            {
+                __hip_atomic_store(&status[tile], LB_AGG | (unsigned long long) run,
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                int64_t p = tile - 1;
+                int spins = 0;
+                while (true)
+                {
+                    const unsigned long long w =
+                        __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long flag = w & ~LB_MASK;
+                    if (flag == 0)
+                    {
+                        if (++spins > LB_SPIN_LIMIT) { atomicExch(err, 1); break; }
+                        __builtin_amdgcn_s_sleep(2);
+                        continue;
+                    }
+                    prefix += (int64_t) (w & LB_MASK);
+                    if (flag == LB_PRE) break;
+                    p--;
+                }
+                __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_prefix = prefix;
+        }
+        __syncthreads();
+        const int64_t prefix = s_prefix;
+        const int64_t total = s_off[FUSED_ROWS];
+        if (sub == 0 && i < g.m) g.Cp[i] = prefix + s_off[team];
+        if (tile == ntiles - 1 && threadIdx.x == 0) g.Cp[g.m] = prefix + total;
+        // contiguous copy of the tile's small-row output
+        for (int64_t e = threadIdx.x; e < total; e += SX_NT)
+        {
+            int lo = 0, hi = FUSED_ROWS - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
+            }
+            if (s_big[lo]) continue;
+            const int pos = (int) (e - s_off[lo]);
+            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[lo])[pos];
+            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[lo][pos];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_sx_fill_u64(unsigned long long *x, int64_t n)
+{
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t) gridDim.x * blockDim.x)
+        x[t] = 0;
+}
+
 static inline unsigned sx_grid(int64_t items, int64_t per_block, int64_t cap)
 {
     int64_t g = (items + per_block - 1) / per_block;
@@ -569,7 +923,7 @@ static inline unsigned sx_grid(int64_t items, int64_t per_block, int64_t cap)
 using SxSym = SR<GM_ADD_ANY, GM_MULT_PAIR, uint8_t>;
 
 template <class S, int PHASE>
-static int sx_launch_phase(SxArgs &g, const int64_t *cnt)
+static int sx_launch_phase(SxArgs &g, const int64_t *cnt, bool skip_ws)
 {
     hipStream_t st = stream();
     const int64_t cap = (int64_t) cu_count() * 32;
@@ -582,7 +936,7 @@ static int sx_launch_phase(SxArgs &g, const int64_t *cnt)
         k_sx_single<S, PHASE><<<sx_grid(cnt[BIN_SINGLE], 4, cap), SX_NT, 0, st>>>(g);
         GM_HIP(hipGetLastError(), "k_sx_single");
     }
-    if (cnt[BIN_WS] > 0)
+    if (cnt[BIN_WS] > 0 && !skip_ws)
     {
         ProfScope ps(names[PHASE][1]);
         k_sx_wave<S, 7, PHASE><<<sx_grid(cnt[BIN_WS], 4, cap), SX_NT, 0, st>>>(g, BIN_WS);
@@ -617,10 +971,11 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     hipStream_t st = stream();
     const int64_t m = A->nrows, n = B->ncols;
     const size_t zs = sizeof(Z);
+    const size_t entry_bytes = sizeof(int64_t) + (S::iso_out ? 0 : zs);
 
     DevBuf Cp, rows, cnt, goff, glog;
     if (!Cp.alloc(sizeof(int64_t) * (m + 1)) || !rows.alloc(sizeof(uint32_t) * SX_NBINS * (m > 0 ? m : 1)) ||
-        !cnt.alloc(sizeof(unsigned long long) * 8) || !goff.alloc(sizeof(int64_t) * (m > 0 ? m : 1)) ||
+        !cnt.alloc(sizeof(unsigned long long) * 16) || !goff.alloc(sizeof(int64_t) * (m > 0 ? m : 1)) ||
         !glog.alloc(sizeof(int32_t) * (m > 0 ? m : 1)))
         return set_error(GM_OUT_OF_MEMORY, "saxpy3: workspace");
 
@@ -641,26 +996,69 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     g.ws_max = (prm && prm->wave_small_max > 0) ? prm->wave_small_max : 64;
     g.wl_max = (prm && prm->wave_large_max > 0) ? prm->wave_large_max : 512;
     g.blk_max = (prm && prm->block_max > 0) ? prm->block_max : 2048;
-    if (g.ws_max > 64) g.ws_max = 64;
+    if (g.ws_max > FMAX) g.ws_max = FMAX;
     if (g.wl_max > 512) g.wl_max = 512;
     if (g.blk_max > 2048) g.blk_max = 2048;
+    const int64_t fs = g.ws_max;        // flops bound of a "small" row
+    const int mode = prm ? (int) prm->mode : 0;
+    // the fused kernel holds C's small rows at their flop bound until the
+    // scan; it is used while that bound stays within a third of HBM
+    const double budget = (double) device_hbm_bytes() / 3.0;
 
-    int64_t hc[8] = {0};
-    if (m > 0)
+    int64_t hc[16] = {0};
+    unsigned long long *hp = (unsigned long long *) pinned_scratch(16 * sizeof(unsigned long long));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    if (m == 0)
     {
-        k_sx_zero<<<1, 64, 0, st>>>(g.cnt, 8);
+        k_sx_fill_u64<<<1, 64, 0, st>>>((unsigned long long *) g.Cp, 1);
+        GM_HIP(hipGetLastError(), "k_sx_fill_u64");
+        C->nrows = m; C->ncols = n; C->nvals = 0;
+        if (!Cp.ptr) return GM_OUT_OF_MEMORY;
+        DevBuf Ci, Cx;
+        if (!Ci.alloc(8) || !Cx.alloc(zs)) return set_error(GM_OUT_OF_MEMORY, "saxpy3: output");
+        GM_HIP(hipStreamSynchronize(st), "saxpy3 sync");
+        C->p = (int64_t *) Cp.release(); C->j = (int64_t *) Ci.release(); C->x = Cx.release();
+        C->iso = S::iso_out ? 1 : 0;
+        return GM_SUCCESS;
+    }
+
+    // 1. max row lengths of A and B, and nnz(A)
+    k_sx_fill_u64<<<1, 64, 0, st>>>(g.cnt, 16);
+    GM_HIP(hipGetLastError(), "k_sx_fill_u64");
+    {
+        ProfScope ps("sx_maxlen");
+        const int64_t lim = m > B->nrows ? m : B->nrows;
+        k_sx_maxlen<<<sx_grid(lim, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
+            A->p, m, B->p, B->nrows, g.cnt + 8);
+        GM_HIP(hipGetLastError(), "k_sx_maxlen");
+    }
+    GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+           "maxlen d2h");
+    GM_HIP(hipMemcpyAsync(hp + 10, A->p + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
+    GM_HIP(hipStreamSynchronize(st), "maxlen sync");
+    const int64_t maxA = (int64_t) hp[8], maxB = (int64_t) hp[9], nnzA = (int64_t) hp[10];
+
+    bool fused = (mode != 2);
+    bool all_small = fused && (double) maxA * (double) maxB <= (double) fs &&
+                     (double) nnzA * (double) maxB * (double) entry_bytes <= budget;
+    int64_t cap_small = all_small ? nnzA * maxB : 0;
+
+    // 2. flop count and binning (skipped when every row is small)
+    if (!all_small)
+    {
         {
             ProfScope ps("sx_flops");
-            k_sx_flops<<<sx_grid(m, SX_NT, (int64_t) cu_count() * 16), SX_NT, 0, st>>>(g);
+            k_sx_flops<<<sx_grid(m, SX_NT / TG, (int64_t) cu_count() * 16), SX_NT, 0, st>>>(g, fs);
             GM_HIP(hipGetLastError(), "k_sx_flops");
         }
-        unsigned long long *hp = (unsigned long long *) pinned_scratch(8 * sizeof(unsigned long long));
-        if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
         GM_HIP(hipMemcpyAsync(hp, g.cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
                "bin counts");
         GM_HIP(hipStreamSynchronize(st), "bin counts sync");
         for (int t = 0; t < 8; t++) hc[t] = (int64_t) hp[t];
+        cap_small = hc[7];
+        if ((double) cap_small * (double) entry_bytes > budget) fused = false;
     }
+
     DevBuf gkeys, gvals, glist;
     const int64_t gslots = hc[SX_NBINS];
     if (gslots > 0)
@@ -671,23 +1069,67 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
                              (long long) gslots);
         g.gkeys = gkeys.as<uint32_t>(); g.gvals = gvals.ptr; g.glist = glist.as<uint64_t>();
     }
-    // symbolic
-    if (m > 0) GM_TRY((sx_launch_phase<SxSym, 0>(g, hc)));
-    GM_TRY(scan_exclusive_i64(g.Cp, g.Cp, m));
+    const bool have_big = hc[BIN_SINGLE] + hc[BIN_WL] + hc[BIN_BLK] + hc[BIN_GLB] > 0;
     int64_t nvals = 0;
-    {
-        int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
-        GM_HIP(hipMemcpyAsync(hp, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");
-        GM_HIP(hipStreamSynchronize(st), "nnz(C) sync");
-        nvals = *hp;
-    }
     DevBuf Ci, Cx;
-    if (!Ci.alloc(sizeof(int64_t) * (nvals > 0 ? nvals : 1)) ||
-        !Cx.alloc(S::iso_out ? zs : zs * (nvals > 0 ? nvals : 1)))
-        return set_error(GM_OUT_OF_MEMORY, "saxpy3: output (%lld entries)", (long long) nvals);
-    g.Ci = Ci.as<int64_t>();
-    g.Cx = Cx.ptr;
-    if (nvals > 0) GM_TRY((sx_launch_phase<S, 1>(g, hc)));
+    if (fused)
+    {
+        // 3. symbolic phase of the big rows only
+        int64_t big_total = 0;
+        if (have_big)
+        {
+            GM_TRY((sx_launch_phase<SxSym, 0>(g, hc, true)));
+            GM_HIP(hipMemcpyAsync(hp + 6, g.cnt + 6, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, st), "big total");
+            GM_HIP(hipStreamSynchronize(st), "big total sync");
+            big_total = (int64_t) hp[6];
+        }
+        const int64_t cap = cap_small + big_total;
+        if (!Ci.alloc(sizeof(int64_t) * (cap > 0 ? cap : 1)) ||
+            !Cx.alloc(S::iso_out ? zs : zs * (cap > 0 ? cap : 1)))
+            return set_error(GM_OUT_OF_MEMORY, "saxpy3: output (%lld entries)", (long long) cap);
+        g.Ci = Ci.as<int64_t>();
+        g.Cx = Cx.ptr;
+        // 4. fused small-row kernel with the look-back scan over all rows
+        const int64_t ntiles = (m + FUSED_ROWS - 1) / FUSED_ROWS;
+        DevBuf status, errf;
+        if (!status.alloc(sizeof(unsigned long long) * (ntiles + 1)) || !errf.alloc(sizeof(int)))
+            return set_error(GM_OUT_OF_MEMORY, "saxpy3: scan status");
+        k_sx_fill_u64<<<sx_grid(ntiles + 1, 256, 1024), 256, 0, st>>>(
+            status.as<unsigned long long>(), ntiles + 1);
+        k_sx_fill_u64<<<1, 64, 0, st>>>((unsigned long long *) g.cnt + 12, 1);
+        GM_HIP(hipGetLastError(), "k_sx_fill_u64");
+        {
+            ProfScope ps("sx_fused");
+            const int64_t grid = ntiles < (int64_t) cu_count() * 8 ? ntiles : (int64_t) cu_count() * 8;
+            k_sx_fused<S><<<(unsigned) grid, SX_NT, 0, st>>>(g, fs, status.as<unsigned long long>(),
+                                                             ntiles, (int *) (g.cnt + 12));
+            GM_HIP(hipGetLastError(), "k_sx_fused");
+        }
+        GM_HIP(hipMemcpyAsync(hp + 11, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");
+        GM_HIP(hipMemcpyAsync(hp + 12, g.cnt + 12, sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+               "fused err");
+        GM_HIP(hipStreamSynchronize(st), "fused sync");
+        if (hp[12] != 0) return set_error(GM_PANIC, "saxpy3: look-back scan did not complete");
+        nvals = (int64_t) hp[11];
+        // 5. numeric phase of the big rows
+        if (have_big && nvals > 0) GM_TRY((sx_launch_phase<S, 1>(g, hc, true)));
+    }
+    else
+    {
+        // symbolic + scan + numeric for every row
+        GM_TRY((sx_launch_phase<SxSym, 0>(g, hc, false)));
+        GM_TRY(scan_exclusive_i64(g.Cp, g.Cp, m));
+        GM_HIP(hipMemcpyAsync(hp + 11, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");
+        GM_HIP(hipStreamSynchronize(st), "nnz(C) sync");
+        nvals = (int64_t) hp[11];
+        if (!Ci.alloc(sizeof(int64_t) * (nvals > 0 ? nvals : 1)) ||
+            !Cx.alloc(S::iso_out ? zs : zs * (nvals > 0 ? nvals : 1)))
+            return set_error(GM_OUT_OF_MEMORY, "saxpy3: output (%lld entries)", (long long) nvals);
+        g.Ci = Ci.as<int64_t>();
+        g.Cx = Cx.ptr;
+        if (nvals > 0) GM_TRY((sx_launch_phase<S, 1>(g, hc, false)));
+    }
     if (S::iso_out)
     {
         k_sx_iso_one<<<1, 1, 0, st>>>(Cx.ptr, (int) zs);

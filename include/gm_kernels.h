@@ -109,13 +109,17 @@ typedef struct
                           // larger: one 256-thread workgroup per entry
 } gm_dot3_params;
 
-// saxpy3 tuning knobs (0 = default).  Row work w = flops + nnz(M(i,:)).
+// saxpy3 tuning knobs (0 = default).  f = flops of a row; t = its table
+// size bound (f, or min(f, |M(i,:)|) for C<M>, at most ncols).
 typedef struct
 {
-    int64_t wave_small_max;   // w <= : one wavefront, 128-slot LDS table (64)
-    int64_t wave_large_max;   // w <= : one wavefront, 1024-slot LDS table (512)
-    int64_t block_max;        // w <= : one workgroup, 4096-slot LDS table (2048)
+    int64_t wave_small_max;   // f <= : "small" row (64): single-pass fused kernel,
+                              //        or one wavefront with a 128-slot table
+    int64_t wave_large_max;   // t <= : one wavefront, 1024-slot LDS table (512)
+    int64_t block_max;        // t <= : one workgroup, 4096-slot LDS table (2048)
                               // larger: workgroup per row, global hash table
+    int64_t mode;             // 0 auto, 1 prefer the single-pass fused kernel
+                              // for small rows, 2 symbolic + numeric only
 } gm_saxpy3_params;
 
 //------------------------------------------------------------------------------

@@ -166,6 +166,8 @@ def gm_saxpy3(handle: str, A: Csr, B: Csr, M: Csr = None, mask_comp=False, mask_
     dM = DevCsr(M, mtype) if M is not None else None
     out = capi.GmCsrOut()
     s = capi.GmSemiring(*trip)
+    if params and len(params) == 3:
+        params = tuple(params) + (0,)
     prm = capi.GmSaxpy3Params(*params) if params else None
     rc = capi.lib().gm_saxpy3(ctypes.byref(out), ctypes.byref(dM.view) if dM else None,
                               mask_comp, mask_struct, ctypes.byref(dA.view),

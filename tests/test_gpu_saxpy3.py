@@ -79,7 +79,9 @@ def test_saxpy3_masked(grb, comp, struct, handle):
     assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what="masked")
 
 
-@pytest.mark.parametrize("params", [(1, 1, 1), (64, 512, 2048), (1, 512, 2048), (64, 64, 64)])
+@pytest.mark.parametrize("params", [(1, 1, 1, 0), (64, 512, 2048, 0), (1, 512, 2048, 0),
+                                    (64, 64, 64, 0), (64, 512, 2048, 2), (16, 512, 2048, 1),
+                                    (64, 64, 64, 2), (8, 16, 32, 1)])
 @pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
                                     "GxB_ANY_SECONDI_INT64"])
 def test_saxpy3_forced_bins(grb, params, handle):
@@ -145,3 +147,37 @@ def test_saxpy3_deterministic(grb):
     r1 = gm_saxpy3("GrB_PLUS_TIMES_SEMIRING_FP64", A, A)
     r2 = gm_saxpy3("GrB_PLUS_TIMES_SEMIRING_FP64", A, A)
     assert np.array_equal(r1.x, r2.x) and np.array_equal(r1.j, r2.j)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("handle", ["GrB_PLUS_TIMES_SEMIRING_FP64", "GxB_ANY_PAIR_BOOL",
+                                    "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_PLUS_PAIR_INT32"])
+def test_saxpy3_fused_vs_two_phase(grb, mode, handle):
+    """The single-pass fused path and the symbolic+numeric path agree with the oracle
+    on a mix of small and big rows (ER rows plus power-law rows)."""
+    print(f"[saxpy3] mode={mode} {handle}", flush=True)
+    rng = np.random.default_rng(40 + mode)
+    vt = _vals(handle)
+    A = power_law_csr(700, 6, rng, values=vt)
+    B = power_law_csr(700, 6, rng, values=vt)
+    name, _ = SR[handle]
+    ref = O.saxpy3(name, A, B)
+    got = gm_saxpy3(handle, A, B, params=(64, 512, 2048, mode))
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=f"mode {mode}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_saxpy3_fused_masked_bfs_shape(grb, mode):
+    """C<!M> = F*A with many single-entry rows (small and hub sources)."""
+    G = W.rmat_sym(11, 16, seed=6)
+    n = G.nrows
+    rng = np.random.default_rng(9)
+    k = 300
+    src = rng.choice(n, size=k, replace=False).astype(np.int64)
+    src[:5] = np.argsort(-np.diff(G.p))[:5]       # include the hubs
+    F = W.Csr(k, n, np.arange(k + 1, dtype=np.int64), src)
+    V = rand_csr(k, n, 0.01, rng, "int64")
+    ref = O.saxpy3("any_secondi_int64", F, G, M=V, mask_comp=True)
+    got = gm_saxpy3("GxB_ANY_SECONDI_INT64", F, G, M=V, mask_comp=True,
+                    params=(64, 512, 2048, mode))
+    assert_csr_equal(got, ref, what=f"bfs mode {mode}")
