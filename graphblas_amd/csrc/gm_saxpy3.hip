@@ -216,10 +216,14 @@ __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__
 // single / wave1024 / block4096 / global bins.
 //------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs)
+__global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs, int list_small)
 {
+    __shared__ unsigned int s_bc[SX_NBINS];
+    __shared__ unsigned long long s_base[SX_NBINS];
+    __shared__ unsigned long long s_red[SX_NT / 64];
     const int sub = threadIdx.x & (TG - 1);
     const int lane = threadIdx.x & 63;
+    unsigned long long fsum = 0;          // flops of this thread's small rows
     for (int64_t base = (int64_t) blockIdx.x * (SX_NT / TG); base < g.m;
          base += (int64_t) gridDim.x * (SX_NT / TG))
     {
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs)
         f = team_sum<int64_t>(f);
         int bin = -1;
         int64_t t = f;
-        int64_t fsmall = 0;
+        if (threadIdx.x < SX_NBINS) s_bc[threadIdx.x] = 0;
         if (i < g.m && sub == 0)
         {
             if (g.mode == MODE_MASK)
@@ -248,40 +252,71 @@ __global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs)
             }
             if (t > g.n) t = g.n;
             if (f == 0 || t == 0) g.Cp[i] = 0;
-            else if (f <= fs) { bin = BIN_WS; fsmall = f; }
+            else if (f <= fs) { bin = list_small ? BIN_WS : -1; fsum += (unsigned long long) f; }
             else if (a1 - a0 == 1) bin = BIN_SINGLE;
             else if (t <= g.wl_max) bin = BIN_WL;
             else if (t <= g.blk_max) bin = BIN_BLK;
             else bin = BIN_GLB;
         }
-        // flops of small rows (upper bound of their output size)
-        unsigned long long wsum = (unsigned long long) fsmall;
-        #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
-        if (lane == 0 && wsum) atomicAdd(&g.cnt[7], wsum);
-        #pragma unroll
-        for (int b = 0; b < SX_NBINS; b++)
+        __syncthreads();
+        // block-aggregated appends: LDS counters, one global atomic per bin
+        unsigned int local = 0;
+        if (bin >= 0) local = atomicAdd(&s_bc[bin], 1u);
+        __syncthreads();
+        if (threadIdx.x < SX_NBINS && s_bc[threadIdx.x] > 0)
+            s_base[threadIdx.x] = atomicAdd(&g.cnt[threadIdx.x], (unsigned long long) s_bc[threadIdx.x]);
+        __syncthreads();
+        if (bin >= 0)
         {
-            const bool take = (bin == b);
-            const unsigned long long mask = __ballot(take);
-            if (mask == 0) continue;
-            const int leader = __ffsll((long long) mask) - 1;
-            unsigned long long slot = 0;
-            if (lane == leader) slot = atomicAdd(&g.cnt[b], (unsigned long long) __popcll(mask));
-            slot = __shfl(slot, leader, 64) + (unsigned long long) __popcll(mask & ((1ull << lane) - 1ull));
-            if (take)
+            const unsigned long long slot = s_base[bin] + local;
+            g.rows[(int64_t) bin * g.m + (int64_t) slot] = (uint32_t) i;
+            if (bin == BIN_GLB)
             {
-                g.rows[(int64_t) b * g.m + (int64_t) slot] = (uint32_t) i;
-                if (b == BIN_GLB)
-                {
-                    // table of 2^ceil(log2(2t)) slots, load factor <= 1/2
-                    const int lg = ilog2_ceil(2 * t);
-                    const unsigned long long sz = 1ull << lg;
-                    g.gtab_off[slot] = (int64_t) atomicAdd(&g.cnt[SX_NBINS], sz);
-                    g.gtab_log2[slot] = lg;
-                }
+                // table of 2^ceil(log2(2t)) slots, load factor <= 1/2
+                const int lg = ilog2_ceil(2 * t);
+                const unsigned long long sz = 1ull << lg;
+                g.gtab_off[slot] = (int64_t) atomicAdd(&g.cnt[SX_NBINS], sz);
+                g.gtab_log2[slot] = lg;
             }
         }
+        __syncthreads();
+    }
+    // flops of small rows: one atomic per workgroup
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) fsum += __shfl_xor(fsum, off, 64);
+    if (lane == 0) s_red[threadIdx.x >> 6] = fsum;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long tsum = 0;
+        for (int w = 0; w < SX_NT / 64; w++) tsum += s_red[w];
+        if (tsum) atomicAdd(&g.cnt[7], tsum);
+    }
+}
+
+// sum of the symbolic counts of the rows in the big bins -> cnt[6]
+__global__ __launch_bounds__(SX_NT) void k_sx_sum_big(SxArgs g)
+{
+    __shared__ unsigned long long s_red[SX_NT / 64];
+    unsigned long long sum = 0;
+    const int bins[4] = {BIN_SINGLE, BIN_WL, BIN_BLK, BIN_GLB};
+    for (int b = 0; b < 4; b++)
+    {
+        const int64_t count = (int64_t) g.cnt[bins[b]];
+        const uint32_t *list = g.rows + (int64_t) bins[b] * g.m;
+        for (int64_t q = (int64_t) blockIdx.x * SX_NT + threadIdx.x; q < count;
+             q += (int64_t) gridDim.x * SX_NT)
+            sum += (unsigned long long) g.Cp[list[q]];
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long tsum = 0;
+        for (int w = 0; w < SX_NT / 64; w++) tsum += s_red[w];
+        if (tsum) atomicAdd(&g.cnt[6], tsum);
     }
 }
 
@@ -306,11 +341,7 @@ __global__ __launch_bounds__(SX_NT) void k_sx_single(SxArgs g)
         const int64_t pm0 = g.Mp ? g.Mp[i] : 0, pm1 = g.Mp ? g.Mp[i + 1] : 0;
         if (PHASE == 0 && g.mode == MODE_NONE)
         {
-            if (lane == 0)
-            {
-                g.Cp[i] = b1 - b0;
-                atomicAdd(&g.cnt[6], (unsigned long long) (b1 - b0));
-            }
+            if (lane == 0) g.Cp[i] = b1 - b0;
             continue;
         }
         const Z av = S::geta((const Z *) g.Ax, e, g.a_iso);
@@ -340,11 +371,7 @@ __global__ __launch_bounds__(SX_NT) void k_sx_single(SxArgs g)
             }
             run += __popcll(bal);
         }
-        if (PHASE == 0 && lane == 0)
-        {
-            g.Cp[i] = run;
-            atomicAdd(&g.cnt[6], (unsigned long long) run);
-        }
+        if (PHASE == 0 && lane == 0) g.Cp[i] = run;
     }
 }
 
@@ -515,11 +542,7 @@ __device__ void sx_row(const SxArgs &g, int64_t i, uint32_t *keys, typename S::Z
         {
             tot = block_sum<int64_t, NT>(nnew, s_red);
         }
-        if (tid == 0)
-        {
-            g.Cp[i] = tot;
-            atomicAdd(&g.cnt[6], (unsigned long long) tot);
-        }
+        if (tid == 0) g.Cp[i] = tot;
         return;
     }
 
@@ -679,17 +702,20 @@ constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 22;
+constexpr int LB_WIN = 16;           // look-back window: LB_WIN x 64 predecessors
 
 template <class S>
-__global__ __launch_bounds__(SX_NT) void k_sx_fused(SxArgs g, int64_t fs,
+__global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                                                     unsigned long long *status, int64_t ntiles,
                                                     int *err)
 {
     using Z = typename S::Z;
-    __shared__ uint64_t s_key[FUSED_ROWS][FMAX];    // sort keys, then output columns (int32)
-    __shared__ Z s_val[FUSED_ROWS][FMAX];           // flop values, then output values
-    __shared__ int64_t s_pre[FUSED_ROWS][TG], s_k[FUSED_ROWS][TG], s_bs[FUSED_ROWS][TG];
-    __shared__ Z s_av[FUSED_ROWS][TG];
+    // per-team rows padded by one element so the 8 teams of a wavefront
+    // start on different LDS banks
+    __shared__ uint64_t s_key[FUSED_ROWS][FMAX + 1];  // sort keys, then output columns (int32)
+    __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
+    __shared__ int64_t s_pre[FUSED_ROWS][TG + 1], s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
+    __shared__ Z s_av[FUSED_ROWS][TG + 1];
     __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
     __shared__ int s_big[FUSED_ROWS];
     __shared__ int64_t s_tile, s_prefix;
@@ -771,25 +797,53 @@ __global__ __launch_bounds__(SX_NT) void k_sx_fused(SxArgs g, int64_t fs,
         nallow = team_sum<int>(nallow);
         if (!big && nallow > 0)
         {
-            // bitonic sort of key[0..P)
-            int P = 2;
-            while (P < f) P <<= 1;
-            wave_sync();
-            for (int kk = 2; kk <= P; kk <<= 1)
+            // bitonic sort of the 64 keys: lane sub holds positions
+            // sub*8 .. sub*8+7 in registers; distances < 8 are register
+            // compare-exchanges, distances >= 8 are width-8 shuffles
+            constexpr int PER_L = FMAX / TG;
+            uint64_t kr[PER_L];
+            #pragma unroll
+            for (int r = 0; r < PER_L; r++) kr[r] = key[sub * PER_L + r];
+            #pragma unroll
+            for (int kk = 2; kk <= FMAX; kk <<= 1)
             {
+                #pragma unroll
                 for (int jj = kk >> 1; jj > 0; jj >>= 1)
                 {
-                    for (int t = sub; t < P / 2; t += TG)
+                    if (jj >= PER_L)
                     {
-                        const int lo = 2 * t - (t & (jj - 1));
-                        const int hi = lo + jj;
-                        const bool asc = (lo & kk) == 0;
-                        const uint64_t a = key[lo], b = key[hi];
-                        if ((a > b) == asc) { key[lo] = b; key[hi] = a; }
+                        const int lmask = jj / PER_L;
+                        const bool lower = (sub & lmask) == 0;
+                        #pragma unroll
+                        for (int r = 0; r < PER_L; r++)
+                        {
+                            const int idx = sub * PER_L + r;
+                            const bool asc = (idx & kk) == 0;
+                            const uint64_t o = (uint64_t) __shfl_xor((unsigned long long) kr[r], lmask, TG);
+                            const uint64_t mn = kr[r] < o ? kr[r] : o, mx = kr[r] < o ? o : kr[r];
+                            kr[r] = (lower == asc) ? mn : mx;
+                        }
                     }
-                    wave_sync();
+                    else
+                    {
+                        #pragma unroll
+                        for (int r = 0; r < PER_L; r++)
+                        {
+                            const int partner = r ^ jj;
+                            if (partner > r)
+                            {
+                                const int idx = sub * PER_L + r;
+                                const bool asc = (idx & kk) == 0;
+                                const uint64_t a = kr[r], b = kr[partner];
+                                if ((a > b) == asc) { kr[r] = b; kr[partner] = a; }
+                            }
+                        }
+                    }
                 }
             }
+            #pragma unroll
+            for (int r = 0; r < PER_L; r++) key[sub * PER_L + r] = kr[r];
+            wave_sync();
             // compress: lane owns sorted positions sub*8 .. sub*8+7
             constexpr int PER = FMAX / TG;
             int32_t hj[PER];
@@ -844,42 +898,83 @@ __global__ __launch_bounds__(SX_NT) void k_sx_fused(SxArgs g, int64_t fs,
             s_big[team] = big;
         }
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x < 64)
         {
+            // wavefront 0: tile offsets, then a look-back over up to 64
+            // predecessors per step (lane l reads tile - 1 - l)
+            const int lane = threadIdx.x;
             int64_t run = 0;
-            for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = run; run += s_cnt[t]; }
-            s_off[FUSED_ROWS] = run;
-            int64_t prefix = 0;
-            if (tile == 0)
+            for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[t];
+            if (lane == 0)
             {
-                __hip_atomic_store(&status[0], LB_PRE | (unsigned long long) run, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                int64_t r2 = 0;
+                for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = r2; r2 += s_cnt[t]; }
+                s_off[FUSED_ROWS] = r2;
+                __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            else
+            int64_t prefix = 0;
+            if (tile > 0)
             {
-                __hip_atomic_store(&status[tile], LB_AGG | (unsigned long long) run,
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                int64_t p = tile - 1;
+                // window of LB_WIN*64 predecessors, position u*64 + lane =
+                // distance - 1; all loads are independent (relaxed: the
+                // counts travel inside the status words themselves)
+                int64_t end = tile - 1;
                 int spins = 0;
                 while (true)
                 {
-                    const unsigned long long w =
-                        __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    const unsigned long long flag = w & ~LB_MASK;
-                    if (flag == 0)
+                    unsigned long long w[LB_WIN];
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
                     {
-                        if (++spins > LB_SPIN_LIMIT) { atomicExch(err, 1); break; }
-                        __builtin_amdgcn_s_sleep(2);
+                        const int64_t idx = end - (int64_t) u * 64 - lane;
+                        w[u] = (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : LB_PRE;
+                    }
+                    int firstp = LB_WIN * 64;
+                    bool zero_before = false;
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
+                    {
+                        const unsigned long long flag = w[u] & ~LB_MASK;
+                        const unsigned long long bp = __ballot(flag == LB_PRE);
+                        const unsigned long long b0 = __ballot(flag == 0);
+                        if (firstp == LB_WIN * 64)
+                        {
+                            const int fp = bp ? (__ffsll((long long) bp) - 1) : 64;
+                            const unsigned long long upto =
+                                (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
+                            if (b0 & upto) zero_before = true;
+                            if (fp < 64) firstp = u * 64 + fp;
+                        }
+                    }
+                    if (zero_before)
+                    {
+                        // a predecessor has not published yet
+                        if (++spins > LB_SPIN_LIMIT)
+                        {
+                            if (lane == 0) atomicExch(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    prefix += (int64_t) (w & LB_MASK);
-                    if (flag == LB_PRE) break;
-                    p--;
+                    int64_t v = 0;
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
+                        if (u * 64 + lane <= firstp) v += (int64_t) (w[u] & LB_MASK);
+                    #pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                    prefix += v;
+                    if (firstp < LB_WIN * 64) break;
+                    end -= LB_WIN * 64;
                 }
-                __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_prefix = prefix;
+            if (lane == 0) s_prefix = prefix;
         }
         __syncthreads();
         const int64_t prefix = s_prefix;
@@ -1048,7 +1143,8 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     {
         {
             ProfScope ps("sx_flops");
-            k_sx_flops<<<sx_grid(m, SX_NT / TG, (int64_t) cu_count() * 16), SX_NT, 0, st>>>(g, fs);
+            k_sx_flops<<<sx_grid(m, SX_NT / TG, (int64_t) cu_count() * 16), SX_NT, 0, st>>>(
+                g, fs, fused ? 0 : 1);
             GM_HIP(hipGetLastError(), "k_sx_flops");
         }
         GM_HIP(hipMemcpyAsync(hp, g.cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
@@ -1056,7 +1152,19 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         GM_HIP(hipStreamSynchronize(st), "bin counts sync");
         for (int t = 0; t < 8; t++) hc[t] = (int64_t) hp[t];
         cap_small = hc[7];
-        if ((double) cap_small * (double) entry_bytes > budget) fused = false;
+        if (fused && (double) cap_small * (double) entry_bytes > budget)
+        {
+            // too large to hold at the flop bound: redo the binning with the
+            // small rows listed for the symbolic + numeric path
+            fused = false;
+            k_sx_fill_u64<<<1, 64, 0, st>>>(g.cnt, 16);
+            k_sx_flops<<<sx_grid(m, SX_NT / TG, (int64_t) cu_count() * 16), SX_NT, 0, st>>>(g, fs, 1);
+            GM_HIP(hipGetLastError(), "k_sx_flops");
+            GM_HIP(hipMemcpyAsync(hp, g.cnt, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+                   "bin counts");
+            GM_HIP(hipStreamSynchronize(st), "bin counts sync");
+            for (int t = 0; t < 8; t++) hc[t] = (int64_t) hp[t];
+        }
     }
 
     DevBuf gkeys, gvals, glist;
@@ -1079,6 +1187,9 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         if (have_big)
         {
             GM_TRY((sx_launch_phase<SxSym, 0>(g, hc, true)));
+            const int64_t nbig = hc[BIN_SINGLE] + hc[BIN_WL] + hc[BIN_BLK] + hc[BIN_GLB];
+            k_sx_sum_big<<<sx_grid(nbig, SX_NT, 512), SX_NT, 0, st>>>(g);
+            GM_HIP(hipGetLastError(), "k_sx_sum_big");
             GM_HIP(hipMemcpyAsync(hp + 6, g.cnt + 6, sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, st), "big total");
             GM_HIP(hipStreamSynchronize(st), "big total sync");
