@@ -155,6 +155,7 @@ def main():
         C = capi.Matrix("int64", L.nrows, L.nrows)
 
         def step():
+            C.clear()      # a fresh, empty C for every call
             capi.mxm_checked(C, Ld, "GxB_PLUS_PAIR_INT64", Ld, Ud, "ST1")
 
         step()

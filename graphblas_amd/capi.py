@@ -268,6 +268,9 @@ class Matrix:
         except Exception:
             pass
 
+    def clear(self):
+        _check(lib().GrB_Matrix_clear(self.h), "GrB_Matrix_clear", self.h)
+
     @property
     def nvals(self) -> int:
         v = ctypes.c_uint64()
