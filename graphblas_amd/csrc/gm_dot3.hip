@@ -8,10 +8,15 @@
 // under the semiring.  Work is organised per mask entry (never per row, so a
 // hub row of M does not serialise):
 //
+//  bitmaps the longest rows of A and B (within a memory budget) get a bit
+//          array over their column range in HBM, plus a per-word rank for
+//          valued semirings.  A dot product whose longer row has a bitmap
+//          costs min(a,b) independent bit probes instead of a merge.
 //  phase1  one thread per entry: row lookup, mask test, sizes a=|A(i,:)|,
-//          b=|B(j,:)|, work = min(a+b, s*ceil(log2(l+1))).  Small work is
-//          done in place by that thread; larger entries are appended
-//          (wavefront-aggregated atomics) to one of three bins.
+//          b=|B(j,:)|, work = min(a,b) with a bitmap, else
+//          min(a+b, s*ceil(log2(l+1))).  Small work is done in place by that
+//          thread; larger entries are appended (wavefront-aggregated atomics)
+//          to one of three bins.
 //  team16 / wave / block: 16, 64 or 256 lanes per entry.  A balanced pair is
 //          split by merge-path diagonals; an unbalanced pair (s*log l < a+b)
 //          is split over the short list, each lane binary-searching the long
@@ -34,6 +39,43 @@ constexpr int D3_P1_TILE = D3_NT * D3_P1_PER_THREAD;
 constexpr int D3_CT_IPT = 8;          // compaction items per thread
 constexpr int D3_CT_TILE = D3_NT * D3_CT_IPT;
 
+// Probe indexes of the long rows of one operand (rows of length >= T).  A row
+// with column range [lo, hi] gets
+//   a bitmap  when it is at most BM_DENSITY times the size of its index list
+//             (bm_eligible): ((hi-lo)>>6)+1 words, and rank[w] (valued
+//             semirings) counts the row's entries before word w;
+//   a hash    otherwise: 2^ceil(log2(2 len)) slots of (column << 32 | position),
+//             linear probing, EMPTY = all ones.
+// off[r] is the row's first 8-byte unit in the shared pool words[].
+constexpr int64_t BM_DENSITY = 8;
+constexpr uint64_t HX_EMPTY = ~0ull;
+
+struct RowBits
+{
+    int64_t T;                    // INT64_MAX: no bitmaps
+    const int64_t *off;
+    const uint64_t *words;
+    const uint32_t *rank;
+};
+
+__host__ __device__ __forceinline__ bool bm_eligible(int64_t T, int64_t len, int64_t lo, int64_t hi)
+{
+    return len >= T && ((hi - lo) >> 6) + 1 <= BM_DENSITY * len;
+}
+
+__device__ __forceinline__ int hx_log2(int64_t len) { return ilog2_ceil(2 * len); }
+
+// 8-byte units of a long row's probe index
+__device__ __forceinline__ int64_t px_units(int64_t len, int64_t lo, int64_t hi)
+{
+    return bm_eligible(0, len, lo, hi) ? ((hi - lo) >> 6) + 1 : ((int64_t) 1 << hx_log2(len));
+}
+
+__device__ __forceinline__ uint32_t hx_hash(uint32_t key, int lg)
+{
+    return (key * 0x9E3779B1u) >> (32 - lg);
+}
+
 struct Dot3Args
 {
     const int64_t *Mp, *Mj;
@@ -51,6 +93,7 @@ struct Dot3Args
     uint64_t *bins;               // 3 bins x mnz, packed (i << 32) | p
     unsigned long long *bin_cnt;  // 3 counters
     int64_t thread_max, team16_max, wave_max;
+    RowBits bA, bB;               // bitmaps of the long rows of A and B
 };
 
 // lane-aggregated append: returns the slot for lanes with pred set.
@@ -68,28 +111,155 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
     return base + (unsigned long long) __popcll(below);
 }
 
-__device__ __forceinline__ int64_t d3_work(int64_t a, int64_t b)
+// One dot product's operands: row ranges, and how to intersect them.
+//   probe 0: merge / binary search (no bitmap on the longer row)
+//   probe 1: the elements of B(j,:) are probed in A(i,:)'s bitmap
+//   probe 2: the elements of A(i,:) are probed in B(j,:)'s bitmap
+struct Pair
+{
+    int64_t pa, pa_end, pb, pb_end;
+    int64_t off, lo, hi;          // probe index of the longer row
+    int probe;
+    int hlg;                      // 0: bitmap, else log2 of the hash size
+    bool empty;                   // no possible common index
+};
+
+__device__ __forceinline__ int64_t d3_work_merge(int64_t a, int64_t b)
 {
     const int64_t s = a < b ? a : b, l = a < b ? b : a;
     const int64_t srch = s * (int64_t) (ilog2_ceil(l + 1) + 1);
     return srch < a + b ? srch : a + b;
 }
 
-// Single-thread dot product of A(i,:) and B(j,:): a linear merge, or a
-// binary trim of the longer list when it is much longer (the reference's
-// rule at GB_AxB_dot_cij.c:538-616, here chosen by estimated cost).
+// fill the pair for A(i,:) . B(j,:); returns the estimated work
+__device__ __forceinline__ int64_t d3_pair(const Dot3Args &g, int64_t i, int64_t j, Pair &P)
+{
+    P.pa = g.Ap[i]; P.pa_end = g.Ap[i + 1];
+    P.pb = g.Bp[j]; P.pb_end = g.Bp[j + 1];
+    P.probe = 0;
+    P.empty = true;
+    const int64_t a = P.pa_end - P.pa, b = P.pb_end - P.pb;
+    if (a == 0 || b == 0) return 0;
+    const int64_t alo = g.Aj[P.pa], ahi = g.Aj[P.pa_end - 1];
+    const int64_t blo = g.Bj[P.pb], bhi = g.Bj[P.pb_end - 1];
+    if (ahi < blo || bhi < alo) return 0;
+    P.empty = false;
+    // probe the shorter row into the longer one's index when it has one
+    if (a >= b && a >= g.bA.T)
+    {
+        P.probe = 1; P.off = g.bA.off[i]; P.lo = alo; P.hi = ahi;
+        P.hlg = bm_eligible(0, a, alo, ahi) ? 0 : hx_log2(a);
+        return b;
+    }
+    if (b >= g.bB.T)
+    {
+        P.probe = 2; P.off = g.bB.off[j]; P.lo = blo; P.hi = bhi;
+        P.hlg = bm_eligible(0, b, blo, bhi) ? 0 : hx_log2(b);
+        return a;
+    }
+    if (a >= g.bA.T)
+    {
+        P.probe = 1; P.off = g.bA.off[i]; P.lo = alo; P.hi = ahi;
+        P.hlg = bm_eligible(0, a, alo, ahi) ? 0 : hx_log2(a);
+        return b;
+    }
+    return d3_work_merge(a, b);
+}
+
+// position of x in the probed row (row start p0), or -1
+template <bool NEED_POS>
+__device__ __forceinline__ int64_t bm_find(const RowBits &bm, const Pair &P, int64_t p0, int64_t x)
+{
+    if (x < P.lo || x > P.hi) return -1;
+    if (P.hlg)
+    {
+        const uint64_t *T = bm.words + P.off;
+        const uint32_t mask = (1u << P.hlg) - 1u;
+        uint32_t h = hx_hash((uint32_t) x, P.hlg);
+        while (true)
+        {
+            const uint64_t e = T[h];
+            if (e == HX_EMPTY) return -1;
+            if ((int64_t) (e >> 32) == x) return p0 + (int64_t) (e & 0xffffffffull);
+            h = (h + 1) & mask;
+        }
+    }
+    const uint64_t d = (uint64_t) (x - P.lo);
+    const uint64_t w = bm.words[P.off + (int64_t) (d >> 6)];
+    const uint64_t bit = 1ull << (d & 63);
+    if (!(w & bit)) return -1;
+    if constexpr (!NEED_POS) return p0;
+    else return p0 + (int64_t) bm.rank[P.off + (int64_t) (d >> 6)] + __popcll(w & (bit - 1));
+}
+
+// probe elements q0, q0+step, ... < q1 of the short row in the long row's
+// index; the iterations are independent, so several probes are in flight
 template <class S>
-__device__ bool dot_serial(const int64_t *__restrict__ Aj, const typename S::Z *__restrict__ Ax,
-                           int a_iso, int64_t pa, int64_t pa_end,
-                           const int64_t *__restrict__ Bj, const typename S::Z *__restrict__ Bx,
-                           int b_iso, int64_t pb, int64_t pb_end, typename S::Z &out)
+__device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P, int64_t q0, int64_t q1,
+                                               int64_t step, bool &e, typename S::Z &z)
 {
     using Z = typename S::Z;
-    const int64_t a = pa_end - pa, b = pb_end - pb;
-    if (a == 0 || b == 0) return false;
-    if (Aj[pa_end - 1] < Bj[pb] || Bj[pb_end - 1] < Aj[pa]) return false;
+    const Z *Ax = (const Z *) g.Ax;
+    const Z *Bx = (const Z *) g.Bx;
+    const bool shortA = (P.probe == 2);
+    const int64_t *Sj = shortA ? g.Aj : g.Bj;
+    const RowBits &bm = shortA ? g.bB : g.bA;
+    const int64_t lp0 = shortA ? P.pb : P.pa;
+    #pragma unroll 4
+    for (int64_t q = q0; q < q1; q += step)
+    {
+        const int64_t x = Sj[q];
+        const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
+        if (pos >= 0)
+        {
+            const int64_t qa = shortA ? q : pos, qb = shortA ? pos : q;
+            Z t = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+            if (!e) { z = t; e = true; }
+            else if constexpr (S::any_min_k) { if (t < z) z = t; }
+            else z = S::upd(z, t);
+            if constexpr (S::is_any && !S::any_min_k) break;
+            if (S::terminal(z) && !S::any_min_k) break;
+        }
+    }
+}
+
+// trim the short row of a probe pair to the overlap of both rows' ranges
+__device__ __forceinline__ void d3_trim_short(const Dot3Args &g, const Pair &P, int64_t &q0, int64_t &q1)
+{
+    const bool shortA = (P.probe == 2);
+    const int64_t *Sj = shortA ? g.Aj : g.Bj;
+    int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
+    const int64_t slo = Sj[s0], shi = Sj[s1 - 1];
+    const int64_t olo = slo > P.lo ? slo : P.lo, ohi = shi < P.hi ? shi : P.hi;
+    if (olo > slo) s0 = lower_bound(Sj, s0, s1, olo);
+    if (ohi < shi) s1 = lower_bound(Sj, s0, s1, ohi + 1);
+    q0 = s0;
+    q1 = s1;
+}
+
+// Single-thread dot product: bitmap probes, or a linear merge / binary trim
+// of the longer list when it is much longer (the reference's rule at
+// GB_AxB_dot_cij.c:538-616, here chosen by estimated cost).
+template <class S>
+__device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
+{
+    using Z = typename S::Z;
     bool e = false;
     Z z = S::identity();
+    if (P.empty) return false;
+    if (P.probe)
+    {
+        if (P.probe == 2) d3_probe_range<S>(g, P, P.pa, P.pa_end, 1, e, z);
+        else d3_probe_range<S>(g, P, P.pb, P.pb_end, 1, e, z);
+        if constexpr (S::plus_pair_wide) e = (z != 0);
+        out = z;
+        return e;
+    }
+    const int64_t *Aj = g.Aj, *Bj = g.Bj;
+    const Z *Ax = (const Z *) g.Ax, *Bx = (const Z *) g.Bx;
+    int64_t pa = P.pa, pb = P.pb;
+    const int64_t pa_end = P.pa_end, pb_end = P.pb_end;
+    const int64_t a = pa_end - pa, b = pb_end - pb;
     const bool trimA = a > 8 * b, trimB = b > 8 * a;
     int64_t ia = Aj[pa], ib = Bj[pb];
     while (true)
@@ -108,7 +278,7 @@ __device__ bool dot_serial(const int64_t *__restrict__ Aj, const typename S::Z *
         }
         else
         {
-            Z t = S::mul(S::geta(Ax, pa, a_iso), S::geta(Bx, pb, b_iso), ia);
+            Z t = S::mul(S::geta(Ax, pa, g.a_iso), S::geta(Bx, pb, g.b_iso), ia);
             if (!e) { z = t; e = true; } else z = S::upd(z, t);
             if constexpr (S::is_any) break;
             if (S::terminal(z)) break;
@@ -125,19 +295,27 @@ __device__ bool dot_serial(const int64_t *__restrict__ Aj, const typename S::Z *
 }
 
 // One lane's share of a team dot product.  lane in [0,G).  Returns the
-// lane's partial (e, z); partials combine in lane order.
+// lane's partial (e, z); partials combine in lane order (= k order).
 template <class S, int G>
-__device__ void dot_lane(int lane, const int64_t *__restrict__ Aj, const typename S::Z *__restrict__ Ax,
-                         int a_iso, int64_t pa, int64_t pa_end,
-                         const int64_t *__restrict__ Bj, const typename S::Z *__restrict__ Bx,
-                         int b_iso, int64_t pb, int64_t pb_end, bool &e, typename S::Z &z)
+__device__ void dot_lane(const Dot3Args &g, const Pair &P, int lane, bool &e, typename S::Z &z)
 {
     using Z = typename S::Z;
     e = false;
     z = S::identity();
+    if (P.empty) return;
+    const int64_t *Aj = g.Aj, *Bj = g.Bj;
+    const Z *Ax = (const Z *) g.Ax, *Bx = (const Z *) g.Bx;
+    const int64_t pa = P.pa, pa_end = P.pa_end, pb = P.pb, pb_end = P.pb_end;
     const int64_t a = pa_end - pa, b = pb_end - pb;
-    if (a == 0 || b == 0) return;
-    if (Aj[pa_end - 1] < Bj[pb] || Bj[pb_end - 1] < Aj[pa]) return;
+    if (P.probe)
+    {
+        // the short row, trimmed to the overlap of both ranges, interleaved
+        // over the lanes (coalesced reads), each probed in the long row's index
+        int64_t q0, q1;
+        d3_trim_short(g, P, q0, q1);
+        d3_probe_range<S>(g, P, q0 + lane, q1, G, e, z);
+        return;
+    }
     const int64_t s = a < b ? a : b, l = a < b ? b : a;
     const bool search = s * (int64_t) (ilog2_ceil(l + 1) + 1) < a + b;
     if (search)
@@ -161,7 +339,7 @@ __device__ void dot_lane(int lane, const int64_t *__restrict__ Aj, const typenam
             if (Lj[lcur] == x)
             {
                 const int64_t qa = shortA ? q : lcur, qb = shortA ? lcur : q;
-                Z t = S::mul(S::geta(Ax, qa, a_iso), S::geta(Bx, qb, b_iso), x);
+                Z t = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
                 if (!e) { z = t; e = true; } else z = S::upd(z, t);
                 if constexpr (S::is_any) break;
                 if (S::terminal(z)) break;
@@ -194,7 +372,7 @@ __device__ void dot_lane(int lane, const int64_t *__restrict__ Aj, const typenam
                 const int64_t kb = Bj[pb + y];
                 if (x > 0 && Aj[pa + x - 1] == kb)
                 {
-                    Z t = S::mul(S::geta(Ax, pa + x - 1, a_iso), S::geta(Bx, pb + y, b_iso), kb);
+                    Z t = S::mul(S::geta(Ax, pa + x - 1, g.a_iso), S::geta(Bx, pb + y, g.b_iso), kb);
                     if (!e) { z = t; e = true; } else z = S::upd(z, t);
                     if constexpr (S::is_any) break;
                     if (S::terminal(z)) break;
@@ -263,21 +441,12 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                 Z z = S::identity();
                 if (g.mask_struct || mask_true(g.Mx, p, g.msize, g.m_iso))
                 {
-                    const int64_t pa = g.Ap[i], pa_end = g.Ap[i + 1];
-                    const int64_t pb = g.Bp[j], pb_end = g.Bp[j + 1];
-                    const int64_t a = pa_end - pa, b = pb_end - pb;
-                    if (a > 0 && b > 0)
+                    Pair P;
+                    const int64_t w = d3_pair(g, i, j, P);
+                    if (!P.empty)
                     {
-                        const int64_t w = d3_work(a, b);
-                        if (w <= g.thread_max)
-                        {
-                            e = dot_serial<S>(g.Aj, (const Z *) g.Ax, g.a_iso, pa, pa_end,
-                                              g.Bj, (const Z *) g.Bx, g.b_iso, pb, pb_end, z);
-                        }
-                        else
-                        {
-                            bin = (w <= g.team16_max) ? 0 : (w <= g.wave_max) ? 1 : 2;
-                        }
+                        if (w <= g.thread_max) e = dot_serial<S>(g, P, z);
+                        else bin = (w <= g.team16_max) ? 0 : (w <= g.wave_max) ? 1 : 2;
                     }
                 }
                 if (bin < 0)
@@ -318,8 +487,9 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_team(Dot3Args g, int bin)
         const int64_t j = g.Mj[p];
         bool e;
         Z z;
-        dot_lane<S, G>(lane, g.Aj, (const Z *) g.Ax, g.a_iso, g.Ap[i], g.Ap[i + 1],
-                       g.Bj, (const Z *) g.Bx, g.b_iso, g.Bp[j], g.Bp[j + 1], e, z);
+        Pair P;
+        (void) d3_pair(g, i, j, P);
+        dot_lane<S, G>(g, P, lane, e, z);
         team_reduce<S, G>(e, z);
         if (lane == 0) d3_store<S>(g, p, e, z);
     }
@@ -342,8 +512,9 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block(Dot3Args g)
         const int64_t j = g.Mj[p];
         bool e;
         Z z;
-        dot_lane<S, D3_NT>(threadIdx.x, g.Aj, (const Z *) g.Ax, g.a_iso, g.Ap[i], g.Ap[i + 1],
-                           g.Bj, (const Z *) g.Bx, g.b_iso, g.Bp[j], g.Bp[j + 1], e, z);
+        Pair P;
+        (void) d3_pair(g, i, j, P);
+        dot_lane<S, D3_NT>(g, P, threadIdx.x, e, z);
         team_reduce<S, 64>(e, z);
         if ((threadIdx.x & 63) == 0) { s_z[wid] = z; s_e[wid] = e ? 1 : 0; }
         __syncthreads();
@@ -463,12 +634,208 @@ __global__ void k_zero_u64(unsigned long long *x, int n)
     if (threadIdx.x < n) x[threadIdx.x] = 0;
 }
 
+//------------------------------------------------------------------------------
+// bitmaps of the long rows
+//------------------------------------------------------------------------------
+
+constexpr int BM_HIST = 64;
+
+// words needed by eligible rows, bucketed by floor(log2(row length))
+__global__ __launch_bounds__(D3_NT) void k_d3_bm_hist(const int64_t *__restrict__ Xp,
+                                                      const int64_t *__restrict__ Xj, int64_t nrows,
+                                                      int64_t minlen, unsigned long long *hist)
+{
+    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
+         r += (int64_t) gridDim.x * D3_NT)
+    {
+        const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
+        if (len < minlen) continue;
+        const int64_t lo = Xj[p0], hi = Xj[p0 + len - 1];
+        const int b = 63 - __clzll((unsigned long long) len);
+        atomicAdd(&hist[b], (unsigned long long) px_units(len, lo, hi));
+    }
+}
+
+// word offsets of the eligible rows (atomic bump) and the list of them
+__global__ __launch_bounds__(D3_NT) void k_d3_bm_assign(const int64_t *__restrict__ Xp,
+                                                        const int64_t *__restrict__ Xj, int64_t nrows,
+                                                        int64_t T, int64_t *off,
+                                                        unsigned long long *wctr,
+                                                        unsigned long long *lctr, uint32_t *list)
+{
+    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
+         r += (int64_t) gridDim.x * D3_NT)
+    {
+        const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
+        if (len < T) continue;
+        const int64_t lo = Xj[p0], hi = Xj[p0 + len - 1];
+        off[r] = (int64_t) atomicAdd(wctr, (unsigned long long) px_units(len, lo, hi));
+        list[atomicAdd(lctr, 1ull)] = (uint32_t) r;
+    }
+}
+
+// One workgroup per listed row: clear its words, then OR in its elements.
+// Elements are sorted, so the lanes holding one word are contiguous: a
+// segmented OR across the wavefront leaves one atomic per word per chunk.
+// rank[w] (when requested) = number of row entries before word w.
+__global__ __launch_bounds__(D3_NT) void k_d3_bm_build(const int64_t *__restrict__ Xp,
+                                                       const int64_t *__restrict__ Xj,
+                                                       const int64_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ list,
+                                                       const unsigned long long *lctr,
+                                                       unsigned long long *words, uint32_t *rank)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t count = (int64_t) *lctr;
+    for (int64_t q = blockIdx.x; q < count; q += gridDim.x)
+    {
+        const int64_t r = list[q];
+        const int64_t p0 = Xp[r], p1 = Xp[r + 1];
+        const int64_t lo = Xj[p0], hi = Xj[p1 - 1];
+        const int64_t o = off[r];
+        if (!bm_eligible(0, p1 - p0, lo, hi))
+        {
+            // hash index: (column << 32 | position)
+            const int lg = hx_log2(p1 - p0);
+            const int64_t ns = (int64_t) 1 << lg;
+            const uint32_t mask = (1u << lg) - 1u;
+            for (int64_t t = threadIdx.x; t < ns; t += D3_NT) words[o + t] = HX_EMPTY;
+            __syncthreads();
+            for (int64_t p = p0 + threadIdx.x; p < p1; p += D3_NT)
+            {
+                const uint64_t key = (uint64_t) Xj[p];
+                const unsigned long long ent = (key << 32) | (uint64_t) (p - p0);
+                uint32_t h = hx_hash((uint32_t) key, lg);
+                while (atomicCAS(&words[o + h], HX_EMPTY, ent) != HX_EMPTY) h = (h + 1) & mask;
+            }
+            __syncthreads();
+            continue;
+        }
+        const int64_t nw = ((hi - lo) >> 6) + 1;
+        for (int64_t w = threadIdx.x; w < nw; w += D3_NT) words[o + w] = 0;
+        __syncthreads();
+        for (int64_t base = p0; base < p1; base += D3_NT)
+        {
+            const int64_t p = base + threadIdx.x;
+            const bool valid = p < p1;
+            int64_t w = -1;
+            unsigned long long bits = 0;
+            int64_t wprev = -1;
+            if (valid)
+            {
+                const int64_t d = Xj[p] - lo;
+                w = d >> 6;
+                bits = 1ull << (d & 63);
+                if (p > p0) wprev = (Xj[p - 1] - lo) >> 6;
+            }
+            #pragma unroll
+            for (int s = 1; s < 64; s <<= 1)
+            {
+                const unsigned long long ob = __shfl_down(bits, s, 64);
+                const int64_t ow = __shfl_down(w, s, 64);
+                if (lane + s < 64 && ow == w) bits |= ob;
+            }
+            const bool head = valid && (w != wprev);
+            if (valid && (head || lane == 0)) atomicOr(&words[o + w], bits);
+            if (head && rank)
+            {
+                const uint32_t pos = (uint32_t) (p - p0);
+                for (int64_t gw = wprev + 1; gw <= w; gw++) rank[o + gw] = pos;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 static inline unsigned grid_for(int64_t work_items, int64_t per_block, int64_t cap)
 {
     int64_t g = (work_items + per_block - 1) / per_block;
     if (g < 1) g = 1;
     if (g > cap) g = cap;
     return (unsigned) g;
+}
+
+// Decide which rows get bitmaps (longest first, within a budget of HBM),
+// then build them.  A and B share bitmaps when they are the same matrix.
+static int dot3_bitmaps(Dot3Args &g, const gm_csr *A, const gm_csr *B, bool need_rank,
+                        int64_t minlen, DevBuf &offA, DevBuf &offB, DevBuf &words, DevBuf &rank,
+                        DevBuf &list, DevBuf &ctr)
+{
+    hipStream_t st = stream();
+    const bool shared = (A->p == B->p && A->j == B->j && A->nrows == B->nrows);
+    if (!ctr.alloc(sizeof(unsigned long long) * (2 * BM_HIST + 8)))
+        return set_error(GM_OUT_OF_MEMORY, "dot3: bitmap counters");
+    unsigned long long *c = ctr.as<unsigned long long>();
+    k_zero_u64<<<1, 64, 0, st>>>(c, 64);
+    k_zero_u64<<<1, 64, 0, st>>>(c + 64, 64);
+    k_zero_u64<<<1, 64, 0, st>>>(c + 128, 8);
+    const int64_t gcap = (int64_t) cu_count() * 8;
+    {
+        ProfScope ps("dot3_bm_hist");
+        k_d3_bm_hist<<<grid_for(A->nrows, D3_NT, gcap), D3_NT, 0, st>>>(A->p, A->j, A->nrows, minlen, c);
+        if (!shared)
+            k_d3_bm_hist<<<grid_for(B->nrows, D3_NT, gcap), D3_NT, 0, st>>>(B->p, B->j, B->nrows,
+                                                                            minlen, c + BM_HIST);
+        GM_HIP(hipGetLastError(), "k_d3_bm_hist");
+    }
+    unsigned long long *h = (unsigned long long *) pinned_scratch(sizeof(unsigned long long) * 2 * BM_HIST);
+    if (!h) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(h, c, sizeof(unsigned long long) * 2 * BM_HIST, hipMemcpyDeviceToHost, st),
+           "bitmap histogram");
+    GM_HIP(hipStreamSynchronize(st), "bitmap histogram sync");
+    const double wbytes = 8.0 + (need_rank ? 4.0 : 0.0);
+    double budget = (double) device_hbm_bytes() / 16.0;
+    if (budget > 16e9) budget = 16e9;
+    // smallest length bucket whose cumulative words fit the budget
+    int b0 = -1;
+    double acc = 0;
+    for (int b = BM_HIST - 1; b >= 0; b--)
+    {
+        const double wb = (double) (h[b] + h[BM_HIST + b]) * wbytes;
+        if (wb == 0) continue;
+        if (acc + wb > budget) break;
+        acc += wb;
+        b0 = b;
+    }
+    if (b0 < 0) return GM_SUCCESS;               // no row qualifies
+    int64_t T = (int64_t) 1 << b0;
+    if (T < minlen) T = minlen;
+    const int64_t total_words = (int64_t) (acc / wbytes) + 1;
+    if (!offA.alloc(sizeof(int64_t) * (A->nrows > 0 ? A->nrows : 1)) ||
+        (!shared && !offB.alloc(sizeof(int64_t) * (B->nrows > 0 ? B->nrows : 1))) ||
+        !words.alloc(sizeof(uint64_t) * total_words) ||
+        (need_rank && !rank.alloc(sizeof(uint32_t) * total_words)) ||
+        !list.alloc(sizeof(uint32_t) * ((A->nrows + (shared ? 0 : B->nrows)) > 0
+                                        ? (A->nrows + (shared ? 0 : B->nrows)) : 1)))
+        return set_error(GM_OUT_OF_MEMORY, "dot3: bitmaps (%lld words)", (long long) total_words);
+    uint32_t *lst = list.as<uint32_t>();
+    // c[128]: word bump shared by A and B; c[129] / c[130]: rows of A / B listed
+    {
+        ProfScope ps("dot3_bm_build");
+        unsigned long long *dw = words.as<unsigned long long>();
+        uint32_t *rk = need_rank ? rank.as<uint32_t>() : nullptr;
+        k_d3_bm_assign<<<grid_for(A->nrows, D3_NT, gcap), D3_NT, 0, st>>>(
+            A->p, A->j, A->nrows, T, offA.as<int64_t>(), c + 128, c + 129, lst);
+        GM_HIP(hipGetLastError(), "k_d3_bm_assign");
+        k_d3_bm_build<<<(unsigned) gcap, D3_NT, 0, st>>>(A->p, A->j, offA.as<int64_t>(), lst, c + 129,
+                                                         dw, rk);
+        GM_HIP(hipGetLastError(), "k_d3_bm_build");
+        if (!shared)
+        {
+            k_d3_bm_assign<<<grid_for(B->nrows, D3_NT, gcap), D3_NT, 0, st>>>(
+                B->p, B->j, B->nrows, T, offB.as<int64_t>(), c + 128, c + 130, lst + A->nrows);
+            GM_HIP(hipGetLastError(), "k_d3_bm_assign");
+            k_d3_bm_build<<<(unsigned) gcap, D3_NT, 0, st>>>(B->p, B->j, offB.as<int64_t>(),
+                                                             lst + A->nrows, c + 130, dw, rk);
+            GM_HIP(hipGetLastError(), "k_d3_bm_build");
+        }
+    }
+    g.bA = RowBits{T, offA.as<int64_t>(), (const uint64_t *) words.ptr,
+                   need_rank ? rank.as<uint32_t>() : nullptr};
+    g.bB = shared ? g.bA
+                  : RowBits{T, offB.as<int64_t>(), (const uint64_t *) words.ptr,
+                            need_rank ? rank.as<uint32_t>() : nullptr};
+    return GM_SUCCESS;
 }
 
 template <class S>
@@ -515,6 +882,14 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         g.team16_max = (prm && prm->team16_max > 0) ? prm->team16_max : 512;
         g.wave_max = (prm && prm->wave_max > 0) ? prm->wave_max : 4096;
         const int64_t cap = (int64_t) cu_count() * 16;
+        g.bA = RowBits{INT64_MAX, nullptr, nullptr, nullptr};
+        g.bB = g.bA;
+        DevBuf bm_offA, bm_offB, bm_words, bm_rank, bm_list, bm_ctr;
+        int64_t minlen = (prm && prm->bitmap_min_len != 0) ? prm->bitmap_min_len : 256;
+        if (A->ncols >= ((int64_t) 1 << 32)) minlen = -1;   // hash keys are 32-bit columns
+        if (minlen > 0)
+            GM_TRY(dot3_bitmaps(g, A, B, !S::pattern, minlen, bm_offA, bm_offB, bm_words, bm_rank,
+                                bm_list, bm_ctr));
         k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
         {
             ProfScope ps("dot3_phase1");

@@ -173,6 +173,9 @@ struct SR
         (ADD == GM_ADD_PLUS && MULT == GM_MULT_PAIR && sizeof(Z) == 8);
     // ANY_PAIR: iso result, value 1 (Source/mxm/GB_AxB__any_pair_iso.c)
     static constexpr bool iso_out = (ADD == GM_ADD_ANY && MULT == GM_MULT_PAIR);
+    // ANY with a positional k: partial results combine to the smallest k,
+    // whatever order the lanes visited the k's in
+    static constexpr bool any_min_k = (ADD == GM_ADD_ANY && MULT == GM_MULT_K);
 
     static __device__ __forceinline__ Z identity()
     {
@@ -218,7 +221,8 @@ __device__ __forceinline__ void combine(bool &e, typename S::Z &z, bool e2, type
 {
     if (!e2) return;
     if (!e) { e = true; z = z2; return; }
-    if constexpr (!S::is_any) z = S::upd(z, z2);
+    if constexpr (S::any_min_k) z = (z2 < z) ? z2 : z;
+    else if constexpr (!S::is_any) z = S::upd(z, z2);
 }
 
 // shuffle helpers for 64-bit and 8-bit payloads

@@ -107,6 +107,8 @@ typedef struct
     int64_t team16_max;   // work <= team16_max: 16 lanes per entry (512)
     int64_t wave_max;     // work <= wave_max: one wavefront per entry (4096)
                           // larger: one 256-thread workgroup per entry
+    int64_t bitmap_min_len;  // rows at least this long may get a bitmap (256);
+                             // < 0: no bitmaps (merge / search only)
 } gm_dot3_params;
 
 // saxpy3 tuning knobs (0 = default).  f = flops of a row; t = its table

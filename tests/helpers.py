@@ -146,6 +146,8 @@ def gm_dot3(handle: str, M: Csr, A: Csr, B: Csr, mtype=8, abtype=None, mask_stru
     dM, dA, dB = DevCsr(M, mtype), DevCsr(A, abtype), DevCsr(B, abtype)
     out = capi.GmCsrOut()
     s = capi.GmSemiring(*trip)
+    if params and len(params) == 3:
+        params = tuple(params) + (0,)
     prm = capi.GmDot3Params(*params) if params else None
     rc = capi.lib().gm_dot3(ctypes.byref(out), ctypes.byref(dM.view), mask_struct,
                             ctypes.byref(dA.view), ctypes.byref(dB.view), ctypes.byref(s),

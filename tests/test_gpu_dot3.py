@@ -60,12 +60,16 @@ def test_dot3_semirings_cabi(grb, handle, shape):
     assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=handle)
 
 
-@pytest.mark.parametrize("params", [(1, 1, 1), (1, 2, 100000), (1, 10 ** 9, 10 ** 9),
-                                    (10 ** 9, 10 ** 9, 10 ** 9), (0, 0, 0)])
+@pytest.mark.parametrize("params", [(1, 1, 1, -1), (1, 2, 100000, -1), (1, 10 ** 9, 10 ** 9, -1),
+                                    (10 ** 9, 10 ** 9, 10 ** 9, -1), (0, 0, 0, 0),
+                                    (1, 1, 1, 1), (1, 2, 100000, 1), (1, 10 ** 9, 10 ** 9, 1),
+                                    (10 ** 9, 10 ** 9, 10 ** 9, 1), (0, 0, 0, 1), (0, 0, 0, 20)])
 @pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
                                     "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_ANY_SECONDI_INT64"])
 def test_dot3_forced_bins(grb, params, handle):
-    """Every bin (thread / team16 / wave / workgroup) gives the same result."""
+    """Every bin (thread / team16 / wave / workgroup), with and without row
+    bitmaps (bitmap_min_len 1: every row probed through a bitmap), gives the
+    oracle's result."""
     print(f"[dot3] forced bins {params} {handle}", flush=True)
     rng = np.random.default_rng(7)
     vt = _vals(handle)
@@ -120,3 +124,19 @@ def test_dot3_deterministic_fp64(grb):
     r1 = gm_dot3("GrB_PLUS_TIMES_SEMIRING_FP64", M, A, A)
     r2 = gm_dot3("GrB_PLUS_TIMES_SEMIRING_FP64", M, A, A)
     assert np.array_equal(r1.x, r2.x)
+
+
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_ANY_SECONDI_INT64",
+                                    "GxB_ANY_PAIR_BOOL"])
+def test_dot3_bitmaps_distinct_operands(grb, handle):
+    """A and B distinct with long rows on both sides (separate bitmap sets)."""
+    rng = np.random.default_rng(99)
+    vt = _vals(handle)
+    A = power_law_csr(600, 40, rng, values=vt)
+    B = power_law_csr(600, 40, rng, values=vt)
+    M = power_law_csr(600, 20, rng, values="int64")
+    name, _ = SR[handle]
+    ref = O.dot3(name, M, A, B)
+    got = gm_dot3(handle, M, A, B, params=(0, 0, 0, 8))
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what="bitmaps")
