@@ -188,6 +188,33 @@ int gm_saxpy3(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct,
               const gm_saxpy3_params *params);
 
 //------------------------------------------------------------------------------
+// batches: many independent requests in one launch sequence (no counterpart
+// in the reference; for callers with many small products)
+//------------------------------------------------------------------------------
+
+// C_r<M_r> = A_r * B_r' for r < nreq, one semiring.  Requests are stacked
+// block-diagonally; C is the stacked result: rows [row_base[r],
+// row_base[r+1]) are request r's, with its column indices offset by
+// col_base[r].  row_base and col_base are caller-allocated host arrays of
+// nreq+1 entries, filled here.
+typedef struct { gm_csr M, A, B; } gm_dot3_request;
+int gm_batch_dot3(gm_csr_out *C, int64_t *row_base, int64_t *col_base,
+                  const gm_dot3_request *reqs, int64_t nreq, bool mask_struct,
+                  const gm_semiring *sr, const gm_dot3_params *params);
+
+// C_r<M_r> = A_r * B (M_r optional, all or none) with one shared B, e.g. many
+// frontiers against one served graph.  Requests are stacked by rows.
+typedef struct { gm_csr M, A; } gm_saxpy3_request;   // M.p == NULL: no mask
+int gm_batch_saxpy3(gm_csr_out *C, int64_t *row_base, const gm_saxpy3_request *reqs,
+                    int64_t nreq, bool mask_comp, bool mask_struct, const gm_csr *B,
+                    const gm_semiring *sr, const gm_saxpy3_params *params);
+
+// materialise the per-request results of a stacked C as separate CSR
+// matrices (out[r], released with gm_free); col_base may be NULL
+int gm_batch_split(gm_csr_out *out, const gm_csr_out *C, const int64_t *row_base,
+                   const int64_t *col_base, int64_t nreq);
+
+//------------------------------------------------------------------------------
 // reduce a value array to a scalar with a monoid (device -> host scalar)
 //------------------------------------------------------------------------------
 
