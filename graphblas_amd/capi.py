@@ -98,6 +98,32 @@ class GmSaxpy3Params(ctypes.Structure):
                 ("block_max", ctypes.c_int64), ("mode", ctypes.c_int64)]
 
 
+class GmDot3Request(ctypes.Structure):
+    _fields_ = [("M", GmCsr), ("A", GmCsr), ("B", GmCsr)]
+
+
+class GmSaxpy3Request(ctypes.Structure):
+    _fields_ = [("M", GmCsr), ("A", GmCsr)]
+
+
+class GmServerConfig(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_int32), ("max_batch_reqs", ctypes.c_int32),
+                ("flop_budget", ctypes.c_int64), ("reserved", ctypes.c_int64)]
+
+
+class GmServerResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("pad", ctypes.c_int32), ("k", ctypes.c_int64),
+                ("nvals", ctypes.c_int64), ("flops", ctypes.c_int64),
+                ("latency_us", ctypes.c_double), ("queue_us", ctypes.c_double),
+                ("batch_id", ctypes.c_int64), ("batch_size", ctypes.c_int64)]
+
+
+class GmServerCounters(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("pending", ctypes.c_int64), ("max_in_flight", ctypes.c_int64),
+                ("mean_batch", ctypes.c_double), ("flops", ctypes.c_double)]
+
+
 class GmProfRecord(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_int64),
                 ("total_ms", ctypes.c_double)]
@@ -197,6 +223,23 @@ def lib():
                        ctypes.c_bool, ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                        ctypes.POINTER(GmSemiring), ctypes.POINTER(GmSaxpy3Params)], ctypes.c_int),
         "gm_reduce_scalar": ([P, I32, I32, P, I64, I32], ctypes.c_int),
+        "gm_batch_dot3": ([ctypes.POINTER(GmCsrOut), P, P, ctypes.POINTER(GmDot3Request), I64,
+                           ctypes.c_bool, ctypes.POINTER(GmSemiring),
+                           ctypes.POINTER(GmDot3Params)], ctypes.c_int),
+        "gm_batch_saxpy3": ([ctypes.POINTER(GmCsrOut), P, ctypes.POINTER(GmSaxpy3Request), I64,
+                             ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr),
+                             ctypes.POINTER(GmSemiring), ctypes.POINTER(GmSaxpy3Params)],
+                            ctypes.c_int),
+        "gm_batch_split": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsrOut), P, P, I64],
+                           ctypes.c_int),
+        "gm_server_create": ([ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
+                              ctypes.POINTER(GmServerConfig)], P),
+        "gm_server_submit": ([P, P, I64, P, P], I64),
+        "gm_server_wait": ([P, I64, ctypes.POINTER(GmServerResult)], ctypes.c_int),
+        "gm_server_fetch": ([P, I64, P, P, P], ctypes.c_int),
+        "gm_server_release": ([P, I64], ctypes.c_int),
+        "gm_server_stats": ([P, ctypes.POINTER(GmServerCounters)], ctypes.c_int),
+        "gm_server_destroy": ([P], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -396,3 +439,72 @@ def prof_get() -> dict:
     recs = (GmProfRecord * 64)()
     n = lib().gm_prof_get(recs, 64)
     return {recs[t].name.decode(): (recs[t].launches, recs[t].total_ms) for t in range(min(n, 64))}
+
+
+class Server:
+    """Served frontier expansion on one resident graph (gm_server_*).
+
+    graph: a gm_csr view whose device arrays outlive the server."""
+
+    def __init__(self, graph: GmCsr, semiring=(4, 4, 8), workers=2, max_batch_reqs=256,
+                 flop_budget=1 << 24):
+        self._graph = graph
+        self._sr = GmSemiring(*semiring)
+        self._zt = semiring[2]
+        cfg = GmServerConfig(workers, max_batch_reqs, flop_budget, 0)
+        self.h = lib().gm_server_create(ctypes.byref(self._graph), ctypes.byref(self._sr),
+                                        ctypes.byref(cfg))
+        if not self.h:
+            raise GrBError(-1, last_error())
+
+    def submit(self, sources, vis=None) -> int:
+        src = np.ascontiguousarray(sources, dtype=np.int64)
+        vp = vj = None
+        if vis is not None:
+            vp = np.ascontiguousarray(vis[0], dtype=np.int64)
+            vj = np.ascontiguousarray(vis[1], dtype=np.int64)
+            if len(vj) == 0:
+                vj = np.zeros(1, dtype=np.int64)
+        rid = lib().gm_server_submit(self.h, src.ctypes.data, len(src),
+                                     vp.ctypes.data if vp is not None else None,
+                                     vj.ctypes.data if vj is not None else None)
+        if rid <= 0:
+            raise GrBError(int(rid), last_error())
+        return rid
+
+    def wait(self, rid) -> GmServerResult:
+        res = GmServerResult()
+        rc = lib().gm_server_wait(self.h, rid, ctypes.byref(res))
+        if rc != 0:
+            raise GrBError(rc, last_error())
+        return res
+
+    def fetch(self, rid, res: GmServerResult):
+        """(row pointers, column indices, values) of a completed request."""
+        dt = {8: np.int64, 6: np.int32, 1: np.uint8, 11: np.float64}[self._zt]
+        p = np.empty(res.k + 1, dtype=np.int64)
+        j = np.empty(max(res.nvals, 1), dtype=np.int64)
+        x = np.empty(max(res.nvals, 1), dtype=dt)
+        rc = lib().gm_server_fetch(self.h, rid, p.ctypes.data, j.ctypes.data, x.ctypes.data)
+        if rc != 0:
+            raise GrBError(rc, last_error())
+        return p, j[:res.nvals], x[:res.nvals]
+
+    def release(self, rid):
+        lib().gm_server_release(self.h, rid)
+
+    def stats(self) -> dict:
+        c = GmServerCounters()
+        lib().gm_server_stats(self.h, ctypes.byref(c))
+        return {f: getattr(c, f) for f, _ in GmServerCounters._fields_}
+
+    def close(self):
+        if self.h:
+            lib().gm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -161,6 +161,26 @@ __global__ void k_bt_gather(const int64_t *src, const int64_t *idx, int64_t *dst
         dst[t] = src[idx[t]];
 }
 
+// positional results (the inner index k) of stacked requests carry the
+// request's inner offset: subtract it, row by row
+__global__ void k_bt_fix_k(const int64_t *Cp, int64_t *Cx, const int64_t *rbase, const int64_t *ibase,
+                           int64_t nparts, int64_t total_rows)
+{
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < total_rows;
+         t += (int64_t) gridDim.x * blockDim.x)
+    {
+        int64_t lo = 0, hi = nparts - 1;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (rbase[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int64_t sub = ibase[lo];
+        if (sub == 0) continue;
+        for (int64_t q = Cp[t]; q < Cp[t + 1]; q++) Cx[q] -= sub;
+    }
+}
+
 // per-request split of a stacked result
 struct SplitPart
 {
@@ -284,7 +304,27 @@ extern "C" int gm_batch_dot3(gm_csr_out *C, int64_t *row_base, int64_t *col_base
     if (!valued) { A.x = nullptr; B.x = nullptr; }
     A.ncols = inner;
     B.ncols = inner;
-    return gm_dot3(C, &M, mask_struct, &A, &B, sr, params);
+    GM_TRY(gm_dot3(C, &M, mask_struct, &A, &B, sr, params));
+    if (sr->mult_op == GM_MULT_K && inner > 0 && nreq > 1)
+    {
+        // positional values: k of the stacked problem = k of the request + its inner base
+        hipStream_t st = stream();
+        std::vector<int64_t> ib(nreq);
+        int64_t acc = 0;
+        for (int64_t r = 0; r < nreq; r++) { ib[r] = acc; acc += reqs[r].A.ncols; }
+        DevBuf drb, dib;
+        if (!drb.alloc(sizeof(int64_t) * nreq) || !dib.alloc(sizeof(int64_t) * nreq))
+            return set_error(GM_OUT_OF_MEMORY, "gm_batch_dot3: positional fix");
+        GM_HIP(hipMemcpyAsync(drb.ptr, row_base, sizeof(int64_t) * nreq, hipMemcpyHostToDevice, st), "fix");
+        GM_HIP(hipMemcpyAsync(dib.ptr, ib.data(), sizeof(int64_t) * nreq, hipMemcpyHostToDevice, st), "fix");
+        int64_t gr = (rows + 255) / 256;
+        if (gr > (int64_t) cu_count() * 16) gr = (int64_t) cu_count() * 16;
+        k_bt_fix_k<<<(unsigned) (gr > 0 ? gr : 1), 256, 0, st>>>(C->p, (int64_t *) C->x, drb.as<int64_t>(),
+                                                               dib.as<int64_t>(), nreq, rows);
+        GM_HIP(hipGetLastError(), "k_bt_fix_k");
+        GM_HIP(hipStreamSynchronize(st), "fix sync");
+    }
+    return GM_SUCCESS;
 }
 
 extern "C" int gm_batch_saxpy3(gm_csr_out *C, int64_t *row_base, const gm_saxpy3_request *reqs,

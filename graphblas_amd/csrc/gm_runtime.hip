@@ -1,11 +1,14 @@
 // gm_runtime.hip: device selection, stream, caching HBM arena, errors,
 // per-kernel event timing and the device exclusive scan.
 //
-// One non-blocking stream at default priority carries every kernel and copy
-// of the library, so stream order alone makes reuse of freed arena blocks
-// safe.  The arena keeps freed blocks in size-keyed free lists (HBM is large;
-// hipMalloc/hipFree are slow and synchronising), and gives them back on
-// allocation failure or gm_arena_trim().
+// Every host thread that calls the library gets its own non-blocking stream
+// at default priority (the initialising thread's is created by gm_init), so
+// concurrent callers -- the request server's workers -- overlap on the device.
+// The arena keeps freed blocks in size-keyed free lists (HBM is large;
+// hipMalloc/hipFree are slow and synchronising), tagged with the stream that
+// freed them: a block is reused at once on its own stream (stream order makes
+// that safe) and after synchronising its stream on another one.  Blocks go
+// back to the driver on allocation failure or gm_arena_trim().
 
 #include "gm_internal.h"
 #include "gm_block.h"
@@ -25,7 +28,9 @@ namespace gm
 static std::mutex g_mu;
 static int g_init_rc = 1;             // 1: not yet initialised
 static int g_device = -1;
-static hipStream_t g_stream = nullptr;
+static hipStream_t g_stream = nullptr;               // the initialising thread's stream
+static thread_local hipStream_t t_stream = nullptr;  // this thread's stream
+static std::vector<hipStream_t> g_thread_streams;    // created for other threads
 static int g_cu = 0;
 static int64_t g_hbm = 0;
 static thread_local char g_err[512] = "";
@@ -47,7 +52,18 @@ int check_hip(hipError_t e, const char *what)
     return set_error(GM_PANIC, "%s: %s", what, hipGetErrorString(e));
 }
 
-hipStream_t stream() { return g_stream; }
+hipStream_t stream()
+{
+    if (t_stream) return t_stream;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return g_stream;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_thread_streams.push_back(s);
+    }
+    t_stream = s;
+    return s;
+}
 int cu_count() { return g_cu; }
 int64_t device_hbm_bytes() { return g_hbm; }
 
@@ -94,7 +110,8 @@ bool csr_ok(const gm_csr *X)
 // arena
 //------------------------------------------------------------------------------
 
-static std::multimap<size_t, void *> g_free;        // size -> block
+struct FreeBlock { void *p; hipStream_t s; };
+static std::multimap<size_t, FreeBlock> g_free;     // size -> block, freeing stream
 static std::unordered_map<void *, size_t> g_live;   // block -> size
 static size_t g_cached_bytes = 0, g_live_bytes = 0;
 static int64_t g_fail_after = -1;
@@ -114,7 +131,7 @@ static size_t round_size(size_t b)
 
 static void trim_locked()
 {
-    for (auto &kv : g_free) (void) hipFree(kv.second);
+    for (auto &kv : g_free) (void) hipFree(kv.second.p);
     g_free.clear();
     g_cached_bytes = 0;
 }
@@ -145,13 +162,14 @@ static hipEvent_t get_event()
 ProfScope::ProfScope(const char *name) : slot(-1)
 {
     if (!g_prof) return;
+    hipStream_t st = stream();      // before the lock: it may create the stream
     std::lock_guard<std::mutex> lk(g_mu);
     int id = -1;
     for (size_t t = 0; t < g_prof_names.size(); t++)
         if (g_prof_names[t] == name) { id = (int) t; break; }
     if (id < 0) { g_prof_names.emplace_back(name); id = (int) g_prof_names.size() - 1; }
     ProfEv ev{id, get_event(), get_event()};
-    (void) hipEventRecord(ev.a, g_stream);
+    (void) hipEventRecord(ev.a, st);
     g_prof_evs.push_back(ev);
     slot = (int) g_prof_evs.size() - 1;
 }
@@ -159,12 +177,13 @@ ProfScope::ProfScope(const char *name) : slot(-1)
 ProfScope::~ProfScope()
 {
     if (slot < 0) return;
+    hipStream_t st = stream();
     std::lock_guard<std::mutex> lk(g_mu);
-    (void) hipEventRecord(g_prof_evs[slot].b, g_stream);
+    (void) hipEventRecord(g_prof_evs[slot].b, st);
 }
 
-static void *g_pinned = nullptr;
-static size_t g_pinned_bytes = 0;
+static thread_local void *g_pinned = nullptr;
+static thread_local size_t g_pinned_bytes = 0;
 
 void *pinned_scratch(size_t bytes)
 {
@@ -234,14 +253,14 @@ int scan_exclusive_i64(const int64_t *d_in, int64_t *d_out, int64_t n)
 {
     if (n <= 0)
     {
-        GM_HIP(hipMemsetAsync(d_out, 0, sizeof(int64_t), g_stream), "scan memset");
+        GM_HIP(hipMemsetAsync(d_out, 0, sizeof(int64_t), stream()), "scan memset");
         return GM_SUCCESS;
     }
     int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nt == 1)
     {
         ProfScope ps("scan_apply");
-        k_scan_apply<<<1, SCAN_NT, 0, g_stream>>>(d_in, n, nullptr, d_out);
+        k_scan_apply<<<1, SCAN_NT, 0, stream()>>>(d_in, n, nullptr, d_out);
         return check_hip(hipGetLastError(), "k_scan_apply");
     }
     DevBuf sums;
@@ -249,13 +268,13 @@ int scan_exclusive_i64(const int64_t *d_in, int64_t *d_out, int64_t n)
         return set_error(GM_OUT_OF_MEMORY, "scan workspace");
     {
         ProfScope ps("scan_reduce");
-        k_scan_reduce<<<(unsigned) nt, SCAN_NT, 0, g_stream>>>(d_in, n, sums.as<int64_t>());
+        k_scan_reduce<<<(unsigned) nt, SCAN_NT, 0, stream()>>>(d_in, n, sums.as<int64_t>());
         GM_HIP(hipGetLastError(), "k_scan_reduce");
     }
     GM_TRY(scan_exclusive_i64(sums.as<int64_t>(), sums.as<int64_t>(), nt));
     {
         ProfScope ps("scan_apply");
-        k_scan_apply<<<(unsigned) nt, SCAN_NT, 0, g_stream>>>(d_in, n, sums.as<int64_t>(), d_out);
+        k_scan_apply<<<(unsigned) nt, SCAN_NT, 0, stream()>>>(d_in, n, sums.as<int64_t>(), d_out);
         GM_HIP(hipGetLastError(), "k_scan_apply");
     }
     return GM_SUCCESS;
@@ -304,6 +323,7 @@ int gm_init(void)
         return g_init_rc;
     }
     g_device = dev;
+    t_stream = g_stream;
     g_init_rc = GM_SUCCESS;
     return GM_SUCCESS;
 }
@@ -324,8 +344,11 @@ void gm_finalize(void)
     if (g_pinned) (void) hipHostFree(g_pinned);
     g_pinned = nullptr;
     g_pinned_bytes = 0;
+    for (auto s2 : g_thread_streams) { (void) hipStreamSynchronize(s2); (void) hipStreamDestroy(s2); }
+    g_thread_streams.clear();
     (void) hipStreamDestroy(g_stream);
     g_stream = nullptr;
+    t_stream = nullptr;
     g_init_rc = 1;
 }
 
@@ -371,12 +394,20 @@ void *gm_malloc(size_t bytes)
         g_fail_after--;
     }
     size_t r = round_size(bytes);
-    auto it = g_free.lower_bound(r);
-    if (it != g_free.end() && it->first <= 2 * r)
+    hipStream_t me = t_stream ? t_stream : g_stream;
+    auto best = g_free.end();
+    for (auto it = g_free.lower_bound(r); it != g_free.end() && it->first <= 2 * r; ++it)
     {
-        void *p = it->second;
-        size_t sz = it->first;
-        g_free.erase(it);
+        if (it->second.s == me) { best = it; break; }
+        if (best == g_free.end()) best = it;
+    }
+    if (best != g_free.end())
+    {
+        void *p = best->second.p;
+        const size_t sz = best->first;
+        // a block freed on another stream may still be in use there
+        if (best->second.s != me) (void) hipStreamSynchronize(best->second.s);
+        g_free.erase(best);
         g_cached_bytes -= sz;
         g_live[p] = sz;
         g_live_bytes += sz;
@@ -387,7 +418,7 @@ void *gm_malloc(size_t bytes)
     if (e != hipSuccess)
     {
         (void) hipGetLastError();
-        (void) hipStreamSynchronize(g_stream);
+        (void) hipDeviceSynchronize();
         trim_locked();
         e = hipMalloc(&p, r);
         if (e != hipSuccess)
@@ -411,7 +442,7 @@ void gm_free(void *p)
     size_t sz = it->second;
     g_live.erase(it);
     g_live_bytes -= sz;
-    g_free.emplace(sz, p);
+    g_free.emplace(sz, FreeBlock{p, t_stream ? t_stream : g_stream});
     g_cached_bytes += sz;
 }
 
@@ -419,7 +450,7 @@ int gm_arena_trim(void)
 {
     if (g_init_rc != GM_SUCCESS) return GM_SUCCESS;
     std::lock_guard<std::mutex> lk(g_mu);
-    (void) hipStreamSynchronize(g_stream);
+    (void) hipDeviceSynchronize();
     trim_locked();
     return GM_SUCCESS;
 }
@@ -436,8 +467,8 @@ int gm_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
     GM_TRY(ensure_init());
     if (bytes == 0) return GM_SUCCESS;
-    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream), "h2d");
-    GM_HIP(hipStreamSynchronize(g_stream), "h2d sync");
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()), "h2d");
+    GM_HIP(hipStreamSynchronize(stream()), "h2d sync");
     return GM_SUCCESS;
 }
 
@@ -445,8 +476,8 @@ int gm_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
     GM_TRY(ensure_init());
     if (bytes == 0) return GM_SUCCESS;
-    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream), "d2h");
-    GM_HIP(hipStreamSynchronize(g_stream), "d2h sync");
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()), "d2h");
+    GM_HIP(hipStreamSynchronize(stream()), "d2h sync");
     return GM_SUCCESS;
 }
 
@@ -454,7 +485,7 @@ int gm_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
     GM_TRY(ensure_init());
     if (bytes == 0) return GM_SUCCESS;
-    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream), "d2d");
+    GM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()), "d2d");
     return GM_SUCCESS;
 }
 
@@ -462,14 +493,14 @@ int gm_memset(void *dst, int value, size_t bytes)
 {
     GM_TRY(ensure_init());
     if (bytes == 0) return GM_SUCCESS;
-    GM_HIP(hipMemsetAsync(dst, value, bytes, g_stream), "memset");
+    GM_HIP(hipMemsetAsync(dst, value, bytes, stream()), "memset");
     return GM_SUCCESS;
 }
 
 int gm_synchronize(void)
 {
     GM_TRY(ensure_init());
-    GM_HIP(hipStreamSynchronize(g_stream), "synchronize");
+    GM_HIP(hipStreamSynchronize(stream()), "synchronize");
     return GM_SUCCESS;
 }
 
@@ -477,7 +508,7 @@ void gm_prof_enable(int on) { g_prof = (on != 0); }
 
 void gm_prof_reset(void)
 {
-    if (g_init_rc == GM_SUCCESS) (void) hipStreamSynchronize(g_stream);
+    if (g_init_rc == GM_SUCCESS) (void) hipDeviceSynchronize();
     std::lock_guard<std::mutex> lk(g_mu);
     for (auto &ev : g_prof_evs) { g_event_pool.push_back(ev.a); g_event_pool.push_back(ev.b); }
     g_prof_evs.clear();
@@ -486,7 +517,7 @@ void gm_prof_reset(void)
 
 int gm_prof_get(gm_prof_record *rec, int cap)
 {
-    if (g_init_rc == GM_SUCCESS) (void) hipStreamSynchronize(g_stream);
+    if (g_init_rc == GM_SUCCESS) (void) hipDeviceSynchronize();
     std::lock_guard<std::mutex> lk(g_mu);
     std::vector<double> tot(g_prof_names.size(), 0.0);
     std::vector<int64_t> cnt(g_prof_names.size(), 0);

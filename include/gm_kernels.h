@@ -215,6 +215,68 @@ int gm_batch_split(gm_csr_out *out, const gm_csr_out *C, const int64_t *row_base
                    const int64_t *col_base, int64_t nreq);
 
 //------------------------------------------------------------------------------
+// served frontier expansion (no counterpart in the reference; for callers
+// that receive many small BFS-step requests against one resident graph)
+//------------------------------------------------------------------------------
+
+// A request is C<!V> = F*A on the server's graph A (square, in HBM): F has
+// one row per source (F(r, src[r]) = 1, iso), V is the request's visited set
+// (k rows; V = F when none is given).  With GxB_ANY_SECONDI_INT64, C(r,i) is
+// src[r] for each newly reached vertex i, the reference's BFS step.  Worker
+// threads, each on its own HIP stream, admit pending requests in arrival
+// order while the batch's flop bound (sum of deg(src)) stays within
+// flop_budget, and run the admitted batch as one stacked saxpy3 call.
+typedef struct
+{
+    int32_t workers;         // worker threads (2)
+    int32_t max_batch_reqs;  // requests per batch at most (256)
+    int64_t flop_budget;     // flop bound per batch (2^24); a larger request runs alone
+    int64_t reserved;
+} gm_server_config;
+
+typedef struct
+{
+    int32_t status;          // GM_SUCCESS or the batch's error code
+    int32_t pad;
+    int64_t k;               // rows (sources) of the request
+    int64_t nvals;           // entries of the request's result
+    int64_t flops;           // its flop bound
+    double latency_us;       // arrival -> result complete
+    double queue_us;         // arrival -> its batch started
+    int64_t batch_id;        // batch that computed it
+    int64_t batch_size;      // requests in that batch
+} gm_server_result;
+
+typedef struct
+{
+    int64_t completed, batches, pending, max_in_flight;
+    double mean_batch;       // requests per batch
+    double flops;            // flop bound of completed requests
+} gm_server_counters;
+
+typedef struct gm_server gm_server;
+
+// graph and sr are captured (graph's device arrays must outlive the server);
+// cfg may be NULL.  Returns NULL on error (gm_last_error).
+gm_server *gm_server_create(const gm_csr *graph, const gm_semiring *sr,
+                            const gm_server_config *cfg);
+// sources: k host vertex ids; vis_p (k+1) / vis_j: optional host CSR of the
+// visited set (both NULL: V = F).  Returns the request id (> 0) or an error
+// code (< 0).
+int64_t gm_server_submit(gm_server *s, const int64_t *sources, int64_t k,
+                         const int64_t *vis_p, const int64_t *vis_j);
+// blocks until the request is complete; fills *out
+int gm_server_wait(gm_server *s, int64_t id, gm_server_result *out);
+// copies the request's rows (k+1 row pointers from 0, nvals indices and
+// values; Cj / Cx may be NULL) to host memory
+int gm_server_fetch(gm_server *s, int64_t id, int64_t *Cp, int64_t *Cj, void *Cx);
+// forgets the request (its HBM result goes with its batch's last request)
+int gm_server_release(gm_server *s, int64_t id);
+int gm_server_stats(gm_server *s, gm_server_counters *st);
+// drains pending requests, joins the workers, frees everything
+void gm_server_destroy(gm_server *s);
+
+//------------------------------------------------------------------------------
 // reduce a value array to a scalar with a monoid (device -> host scalar)
 //------------------------------------------------------------------------------
 

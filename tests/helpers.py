@@ -186,30 +186,12 @@ def gm_saxpy3(handle: str, A: Csr, B: Csr, M: Csr = None, mask_comp=False, mask_
 # batched calls
 # ---------------------------------------------------------------------------
 
-class GmDot3Request(ctypes.Structure):
-    _fields_ = [("M", capi.GmCsr), ("A", capi.GmCsr), ("B", capi.GmCsr)]
-
-
-class GmSaxpy3Request(ctypes.Structure):
-    _fields_ = [("M", capi.GmCsr), ("A", capi.GmCsr)]
+GmDot3Request = capi.GmDot3Request
+GmSaxpy3Request = capi.GmSaxpy3Request
 
 
 def _bind_batch():
-    L = capi.lib()
-    P = ctypes.c_void_p
-    L.gm_batch_dot3.argtypes = [ctypes.POINTER(capi.GmCsrOut), P, P, ctypes.POINTER(GmDot3Request),
-                                ctypes.c_int64, ctypes.c_bool, ctypes.POINTER(capi.GmSemiring),
-                                ctypes.POINTER(capi.GmDot3Params)]
-    L.gm_batch_dot3.restype = ctypes.c_int
-    L.gm_batch_saxpy3.argtypes = [ctypes.POINTER(capi.GmCsrOut), P, ctypes.POINTER(GmSaxpy3Request),
-                                  ctypes.c_int64, ctypes.c_bool, ctypes.c_bool,
-                                  ctypes.POINTER(capi.GmCsr), ctypes.POINTER(capi.GmSemiring),
-                                  ctypes.POINTER(capi.GmSaxpy3Params)]
-    L.gm_batch_saxpy3.restype = ctypes.c_int
-    L.gm_batch_split.argtypes = [ctypes.POINTER(capi.GmCsrOut), ctypes.POINTER(capi.GmCsrOut), P, P,
-                                 ctypes.c_int64]
-    L.gm_batch_split.restype = ctypes.c_int
-    return L
+    return capi.lib()
 
 
 def slice_stacked(C: Csr, r0, r1, col_sub=0) -> Csr:

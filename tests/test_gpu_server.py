@@ -1,0 +1,114 @@
+"""Served frontier expansion (gm_server_*) on the GPU against the oracle.
+
+Each request C<!V> = F*A (ANY_SECONDI_INT64, C(r,i) = src[r]) must give the
+oracle's result bit-exactly whether it ran alone or stacked with others in a
+batch, with one or several worker threads.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from graphblas_amd import capi
+from graphblas_amd import workloads as W
+from helpers import DevCsr, assert_csr_equal, rand_csr
+
+pytestmark = pytest.mark.gpu
+
+ANY_SECONDI = (4, 4, 8)
+
+
+def _requests(n, nreq, seed, kmax=64, vis_frac=0.5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nreq):
+        k = int(rng.integers(1, kmax + 1))
+        src = rng.choice(n, size=k, replace=False).astype(np.int64)
+        V = None
+        if rng.random() < vis_frac:
+            V = rand_csr(k, n, 0.01, rng)
+        out.append((src, V))
+    return out
+
+
+def _check(srv, G, reqs, ids):
+    results = []
+    for (src, V), rid in zip(reqs, ids):
+        res = srv.wait(rid)
+        assert res.status == 0, capi.last_error()
+        p, j, x = srv.fetch(rid, res)
+        k = len(src)
+        F = W.Csr(k, G.nrows, np.arange(k + 1, dtype=np.int64), src)
+        Vm = V if V is not None else F
+        ref = O.saxpy3("any_secondi_int64", F, G, M=Vm, mask_comp=True)
+        assert_csr_equal(W.Csr(k, G.ncols, p, j, x), ref, what=f"request {rid}")
+        results.append(res)
+        srv.release(rid)
+    return results
+
+
+@pytest.mark.parametrize("workers,budget", [(1, 1 << 24), (2, 1 << 24), (2, 20000), (3, 1)])
+def test_server_matches_oracle(grb, workers, budget):
+    print(f"[server] workers={workers} budget={budget}", flush=True)
+    G = W.rmat_sym(12, 16, seed=4)
+    dG = DevCsr(G, 1)
+    srv = capi.Server(dG.view, ANY_SECONDI, workers=workers, flop_budget=budget)
+    try:
+        reqs = _requests(G.nrows, 80, seed=workers * 7 + budget % 5)
+        ids = [srv.submit(src, None if V is None else (V.p, V.j)) for src, V in reqs]
+        res = _check(srv, G, reqs, ids)
+        st = srv.stats()
+        assert st["completed"] == len(reqs)
+        assert st["pending"] == 0
+        assert sum(1 for r in res if r.latency_us >= r.queue_us >= 0) == len(reqs)
+        if budget == 1:
+            assert all(r.batch_size == 1 for r in res)   # every request over budget runs alone
+    finally:
+        srv.close()
+        dG.free()
+
+
+def test_server_alone_equals_mixed(grb):
+    """A request's rows are the same whether it ran alone or in a batch."""
+    G = W.rmat_sym(11, 16, seed=9)
+    dG = DevCsr(G, 1)
+    reqs = _requests(G.nrows, 40, seed=3)
+    try:
+        alone = capi.Server(dG.view, ANY_SECONDI, workers=1, max_batch_reqs=1)
+        mixed = capi.Server(dG.view, ANY_SECONDI, workers=2, max_batch_reqs=64)
+        ia = [alone.submit(s, None if V is None else (V.p, V.j)) for s, V in reqs]
+        im = [mixed.submit(s, None if V is None else (V.p, V.j)) for s, V in reqs]
+        batched = 0
+        for a, m in zip(ia, im):
+            ra, rm = alone.wait(a), mixed.wait(m)
+            assert ra.batch_size == 1
+            batched += rm.batch_size > 1
+            pa, ja, xa = alone.fetch(a, ra)
+            pm, jm, xm = mixed.fetch(m, rm)
+            assert np.array_equal(pa, pm) and np.array_equal(ja, jm) and np.array_equal(xa, xm)
+        alone.close()
+        mixed.close()
+    finally:
+        dG.free()
+
+
+def test_server_plus_times_fp64_and_bad_source(grb):
+    """Another semiring (PLUS_TIMES fp64 on a weighted graph: C(r,:) = A(src,:)
+    outside V) and the host checks of submit."""
+    rng = np.random.default_rng(1)
+    G = W.rmat_sym(10, 8, seed=2)
+    G.x = rng.random(G.nnz)
+    dG = DevCsr(G, 11)
+    srv = capi.Server(dG.view, (1, 1, 11), workers=2)
+    try:
+        with pytest.raises(capi.GrBError):
+            srv.submit(np.array([G.nrows], dtype=np.int64))
+        src = np.array([0, 5, 17], dtype=np.int64)
+        rid = srv.submit(src)
+        res = srv.wait(rid)
+        p, j, x = srv.fetch(rid, res)
+        F = W.Csr(3, G.nrows, np.arange(4, dtype=np.int64), src, np.ones(3))
+        ref = O.saxpy3("plus_times_fp64", F, G, M=F, mask_comp=True)
+        assert_csr_equal(W.Csr(3, G.ncols, p, j, x), ref, fp_tol=0.0, what="plus_times")
+    finally:
+        srv.close()
+        dG.free()
