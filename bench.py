@@ -31,6 +31,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0     # MI355X published HBM3E bandwidth
 
+# HBM bytes per launch of the dominant kernel(s), measured by two rocprofv3
+# counter runs (FETCH_SIZE, WRITE_SIZE; each scaled by what the counter reads
+# for a 1 GiB device copy) and kept as text summaries beside the kernel trace
+PROFILE_SUMMARY = {"saxpy3-er20": "profiles/sx_fused_saxpy3_er20.txt",
+                   "dot3-rmat22": "profiles/dot3_tricount_rmat22.txt"}
+
+
+def measured_traffic(workload):
+    """(bytes per launch, summary file) from the committed counter summary, or (None, None)."""
+    rel = PROFILE_SUMMARY.get(workload)
+    if rel is None or not os.path.exists(os.path.join(ROOT, rel)):
+        return None, None
+    for line in open(os.path.join(ROOT, rel)):
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1]), rel
+    return None, None
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -206,8 +223,11 @@ def main():
     out["mults_per_call"] = mults
     out["alg_bytes_per_call"] = balg
     out["alg_gbs_per_call"] = balg / (ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.workload)
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                       "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                       "traffic_source": traffic_src,
+                       "traffic_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
                        "kernel": dom, "kernel_avg_ms": avg_ms,
                        "kernel_alg_bytes": balg}
     out["kernels_ms_per_step"] = {k: v[1] / args.steps for k, v in sorted(kern.items())}

@@ -424,7 +424,7 @@ def device_info() -> dict:
         raise GrBError(rc, last_error())
     return {"name": info.name.decode(), "arch": info.arch.decode(), "cu_count": info.cu_count,
             "lds_per_block": info.lds_per_block, "hbm_bytes": info.hbm_bytes,
-            "hbm_peak_gbs": info.hbm_peak_gbs}
+            "hbm_peak_gbs_hip_attrs": info.hbm_peak_gbs}
 
 
 def prof_enable(on=True):

@@ -5,8 +5,9 @@
 // This is the thin layer between the C host library (GrB_mxm and friends,
 // include/GraphBLAS.h) and the hand-written HIP kernels.  Every entry point is
 // plain C: device pointers, sizes and small POD structs.  All device memory
-// named here lives in HBM of the current HIP device; the library owns one
-// non-blocking stream (default priority) and a caching device arena.
+// named here lives in HBM of the current HIP device; the library gives each
+// host thread that calls it its own non-blocking stream (default priority)
+// and shares one caching device arena between them.
 //
 // Reference interfaces replaced (SuiteSparse:GraphBLAS v9.3.1):
 //   gm_dot3_branch / gm_dot3     <- GB_cuda_AxB_dot3_branch / GB_cuda_AxB_dot3
@@ -140,7 +141,9 @@ typedef struct
     int32_t cu_count;
     int32_t lds_per_block;
     int64_t hbm_bytes;
-    double hbm_peak_gbs;    // from memory clock and bus width
+    double hbm_peak_gbs;    // 2 x memory clock x bus width as HIP reports them; on
+                            // MI355X this reads 4096, half the published 8 TB/s
+                            // HBM3E rate, so rooflines use the published figure
 } gm_device_info;
 int gm_get_device_info(gm_device_info *info);
 
