@@ -55,7 +55,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="saxpy3-er20",
-                    choices=["saxpy3-er20", "dot3-rmat14", "dot3-rmat22"])
+                    choices=["saxpy3-er20", "dot3-rmat14", "dot3-rmat22", "served-rmat20",
+                             "batched-4096"])
+    ap.add_argument("--requests", type=int, default=4096,
+                    help="served workload: number of requests")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="served workload: Poisson arrival rate in requests/s (0: half the "
+                         "measured saturation throughput of the served path)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -119,6 +125,273 @@ def cpu_baseline(fn, budget_s, unit_per_call):
     return unit_per_call * reps / el, O.num_threads(), reps, el
 
 
+def served_requests(n, nreq, seed):
+    """nreq BFS-step frontiers on an n-vertex graph: k log-uniform in [1, 256]
+    distinct uniform sources per request."""
+    from graphblas_amd import workloads as W
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for r in range(nreq):
+        k = W.loguniform_int(seed, r, 1, 256)
+        reqs.append(np.sort(rng.choice(n, size=k, replace=False)).astype(np.int64))
+    return reqs
+
+
+def run_server(capi, W, view, reqs, rate, workers, max_batch, budget, seed=5):
+    """Drive reqs through a fresh server (Poisson arrivals at the given rate,
+    or all at once when rate <= 0) with the C++ load generator; returns
+    (wall seconds, results, server stats)."""
+    srv = capi.Server(view, (4, 4, 8), workers=workers, max_batch_reqs=max_batch,
+                      flop_budget=budget)
+    arrive = None
+    if rate > 0:
+        rng = np.random.default_rng(seed)
+        arrive = np.cumsum(rng.exponential(1e6 / rate, len(reqs)))
+    res, wall = W.server_drive(srv, reqs, arrive)
+    st = srv.stats()
+    srv.close()
+    return wall, res, st
+
+
+def fetch_all(capi, view, reqs, workers, max_batch, budget):
+    """Every request's rows (untimed; for comparing two configurations)."""
+    srv = capi.Server(view, (4, 4, 8), workers=workers, max_batch_reqs=max_batch,
+                      flop_budget=budget)
+    ids = [srv.submit(s) for s in reqs]
+    rows = []
+    for i in ids:
+        r = srv.wait(i)
+        rows.append(srv.fetch(i, r))
+        srv.release(i)
+    srv.close()
+    return rows
+
+
+def bench_served(args, capi, W):
+    """Served frontier expansion on a resident RMAT scale-20 graph."""
+    G = W.rmat_sym(20, 16, seed=1)
+    dG = capi.DeviceCsr(G, 1)
+    deg = np.diff(G.p)
+    reqs = served_requests(G.nrows, args.requests, seed=11)
+    flops = np.array([int(deg[s].sum()) for s in reqs], dtype=np.int64)
+    total_flops = int(flops.sum())
+    budget = 1 << 22
+    # every result is the same served (mixed into batches) and alone
+    rows_srv = fetch_all(capi, dG.view, reqs, 2, 256, budget)
+    rows_pl = fetch_all(capi, dG.view, reqs, 1, 1, budget)
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+               for a, b in zip(rows_srv, rows_pl))
+    del rows_srv, rows_pl
+    # warm-up, then saturation throughput: every request offered at once
+    run_server(capi, W, dG.view, reqs[:256], 0, 2, 256, budget)
+    t_srv, r_srv, st_srv = run_server(capi, W, dG.view, reqs, 0, 2, 256, budget)
+    t_pl, r_pl, st_pl = run_server(capi, W, dG.view, reqs, 0, 1, 1, budget)
+    sat = len(reqs) / t_srv
+    rate = args.rate if args.rate > 0 else 0.5 * sat
+    # open loop: Poisson arrivals at the fixed rate
+    t_po, r_po, st_po = run_server(capi, W, dG.view, reqs, rate, 2, 256, budget)
+    t_pp, r_pp, st_pp = run_server(capi, W, dG.view, reqs, rate, 1, 1, budget)
+
+    def pct(res):
+        lat = np.array([r.latency_us for r in res]) * 1e-3
+        return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    p50, p99 = pct(r_po)
+    q50, q99 = pct(r_pp)
+    out = {"metric": "served frontier expansion: semiring mults/s (saturation) with p50/p99 ms "
+                     "at a fixed Poisson rate",
+           "unit": "mults/s", "dtype": "int64",
+           "config": {"workload": "served BFS steps C<!V>=F*A, any_secondi int64, RMAT scale 20 "
+                                  "ef 16 bool resident in HBM; k log-uniform [1,256] sources per "
+                                  "request, V = F (first step)",
+                      "n": G.nrows, "nnz": G.nnz, "requests": len(reqs),
+                      "flop_budget": budget, "workers": 2, "max_batch_reqs": 256},
+           "value": total_flops / t_srv, "ms_per_step": t_srv * 1e3 / len(reqs),
+           "requests_per_s": sat,
+           "plain": {"requests_per_s": len(reqs) / t_pl, "mults_per_s": total_flops / t_pl,
+                     "what": "one saxpy3 launch sequence per request (batch size 1, one worker)"},
+           "served_vs_plain": (len(reqs) / t_srv) / (len(reqs) / t_pl),
+           "results_identical_served_vs_plain": bool(same),
+           "mean_batch": st_srv["mean_batch"],
+           "poisson": {"rate_per_s": rate, "p50_ms": p50, "p99_ms": p99,
+                       "achieved_per_s": len(reqs) / t_po, "mean_batch": st_po["mean_batch"],
+                       "max_in_flight": st_po["max_in_flight"],
+                       "plain_p50_ms": q50, "plain_p99_ms": q99,
+                       "plain_achieved_per_s": len(reqs) / t_pp},
+           "total_mults": total_flops,
+           "roofline": None}
+    if not args.no_cpu:
+        # oracle on a stacked sample of the requests (C<!F> = F*A)
+        nr = max(1, len(reqs) // 8)
+        srcs = np.concatenate(reqs[:nr])
+        F = W.Csr(len(srcs), G.nrows, np.arange(len(srcs) + 1, dtype=np.int64), srcs)
+        unit = int(flops[:nr].sum())
+
+        def cpu_fn():
+            import pyoracle as O
+            O.saxpy3("any_secondi_int64", F, G, M=F, mask_comp=True)
+        v, cores, reps, el = cpu_baseline(cpu_fn, args.cpu_seconds, unit)
+        out["cpu_baseline"] = {"value": v, "unit": "mults/s", "cores": cores, "kind": "port",
+                               "sample": f"first {nr} requests stacked into one C<!F>=F*A: "
+                                         f"{reps} calls in {el:.1f} s"}
+    else:
+        out["cpu_baseline"] = None
+    dG.free()
+    return out
+
+
+BATCH_SR = {"plus_pair_int32": (1, 3, 6), "min_plus_int32": (2, 2, 6)}
+
+
+def bench_batched(args, capi, W):
+    """4096 uneven RMAT graphs, C<A> = A*A' each (dot3), plus_pair int32 or
+    min_plus int32: the batched path (one gm_batch_dot3 per semiring) against
+    the plain path (one gm_dot3 per request)."""
+    import ctypes
+    L = capi.lib()
+    nreq = args.requests
+    specs = W.batch_specs(nreq, seed=1)
+    t0 = time.perf_counter()
+    db = W.DeviceBatch(specs)
+    t_gen = time.perf_counter() - t0
+    groups = {name: [g for g, s in enumerate(specs) if s.semiring == name] for name in BATCH_SR}
+    mask_struct = False                      # C<A>: the mask is A's (nonzero) values
+
+    def plain(check):
+        sums = {}
+        for g, s in enumerate(specs):
+            out = capi.GmCsrOut()
+            sr = capi.GmSemiring(*BATCH_SR[s.semiring])
+            v = db.views[g]
+            rc = L.gm_dot3(ctypes.byref(out), ctypes.byref(v), mask_struct, ctypes.byref(v),
+                           ctypes.byref(v), ctypes.byref(sr), None)
+            if rc != 0:
+                raise RuntimeError(f"gm_dot3 request {g}: {capi.last_error()}")
+            if check:
+                sums[g] = stacked_sums(L, out, 0, s.n, 0)
+            for ptr in (out.p, out.j, out.x):
+                L.gm_free(ptr)
+        return sums
+
+    def batched(check):
+        sums = {}
+        for name, idx in groups.items():
+            if not idx:
+                continue
+            reqs = (capi.GmDot3Request * len(idx))()
+            for r, g in enumerate(idx):
+                v = db.views[g]
+                reqs[r] = capi.GmDot3Request(v, v, v)
+            rb = np.zeros(len(idx) + 1, dtype=np.int64)
+            cb = np.zeros(len(idx) + 1, dtype=np.int64)
+            out = capi.GmCsrOut()
+            sr = capi.GmSemiring(*BATCH_SR[name])
+            rc = L.gm_batch_dot3(ctypes.byref(out), rb.ctypes.data, cb.ctypes.data, reqs, len(idx),
+                                 mask_struct, ctypes.byref(sr), None)
+            if rc != 0:
+                raise RuntimeError(f"gm_batch_dot3 {name}: {capi.last_error()}")
+            if check:
+                for r, g in enumerate(idx):
+                    sums[g] = stacked_sums(L, out, int(rb[r]), int(rb[r + 1]), int(cb[r]))
+            for ptr in (out.p, out.j, out.x):
+                L.gm_free(ptr)
+        return sums
+
+    # untimed passes: results of both paths, compared request by request
+    ref = plain(True)
+    got = batched(True)
+    same = all(ref[g] == got[g] for g in range(nreq))
+    # total semiring multiplies: common neighbours over all mask entries
+    mults = 0
+    for g in range(nreq):
+        out = capi.GmCsrOut()
+        sr = capi.GmSemiring(1, 3, 8)
+        v = db.views[g]
+        rc = L.gm_dot3(ctypes.byref(out), ctypes.byref(v), True, ctypes.byref(v), ctypes.byref(v),
+                       ctypes.byref(sr), None)
+        if rc != 0:
+            raise RuntimeError(capi.last_error())
+        r = ctypes.c_int64()
+        L.gm_reduce_scalar(ctypes.byref(r), 1, 8, out.x, out.nvals, out.iso)
+        mults += r.value
+        for ptr in (out.p, out.j, out.x):
+            L.gm_free(ptr)
+    L.gm_synchronize()
+    t0 = time.perf_counter()
+    plain(False)
+    L.gm_synchronize()
+    t_plain = time.perf_counter() - t0
+    steps = max(1, args.steps)
+    capi.prof_reset()
+    capi.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batched(False)
+    L.gm_synchronize()
+    t_b = (time.perf_counter() - t0) / steps
+    capi.prof_enable(False)
+    kern = capi.prof_get()
+    nnz = int(db.nzbase[-1])
+    ns = np.array([s.n for s in specs])
+    out = {"metric": "batched uneven mxm: semiring mults/s, batched vs plain one-call-per-request",
+           "unit": "mults/s", "dtype": "int32",
+           "config": {"workload": f"{nreq} RMAT graphs (n log-uniform [512, 65536], edge factor "
+                                  "uniform [4, 32], a/b/c = 0.57/0.19/0.19, int32 weights [1,100]);"
+                                  " C<A>=A*A' by dot3 with plus_pair int32 or min_plus int32 (1/2 "
+                                  "each)", "requests": nreq, "nnz_total": nnz,
+                      "n_total": int(ns.sum()),
+                      "per_semiring": {k: len(v) for k, v in groups.items()},
+                      "method": "gm_batch_dot3 (block-diagonal stacking, one binned dot3 per "
+                                "semiring) vs gm_dot3 per request"},
+           "value": mults / t_b, "ms_per_step": t_b * 1e3, "steps": steps,
+           "total_mults": int(mults),
+           "plain": {"ms": t_plain * 1e3, "mults_per_s": mults / t_plain,
+                     "requests_per_s": nreq / t_plain},
+           "batched_vs_plain": t_plain / t_b,
+           "results_identical_batched_vs_plain": bool(same),
+           "generate_ms": t_gen * 1e3,
+           "kernels_ms_per_step": {k: v[1] / steps for k, v in sorted(kern.items())},
+           "roofline": None}
+    if not args.no_cpu:
+        import pyoracle as O
+        hosts, units = [], 0
+        for g in range(nreq):
+            if specs[g].n <= 4096:
+                A = W.batch_graph_host(specs[g])
+                c = O.dot3("plus_pair_int64", A, A, A)
+                hosts.append((specs[g].semiring, A))
+                units += int(c.x.sum())
+            if len(hosts) >= 64:
+                break
+
+        def cpu_fn():
+            for name, A in hosts:
+                O.dot3(name, A, A, A, mask_struct=False)
+        v, cores, reps, el = cpu_baseline(cpu_fn, args.cpu_seconds, units)
+        out["cpu_baseline"] = {"value": v, "unit": "mults/s", "cores": cores, "kind": "port",
+                               "sample": f"the first {len(hosts)} requests with n <= 4096, one "
+                                         f"oracle dot3 each: {reps} passes in {el:.1f} s"}
+    else:
+        out["cpu_baseline"] = None
+    db.free()
+    return out
+
+
+def stacked_sums(L, out, r0, r1, col_base):
+    """(nnz, sum of column indices - col_base, wrapped int32 sum of values) of
+    rows [r0, r1) of a CSR result in HBM."""
+    import ctypes
+    pp = np.zeros(2, dtype=np.int64)
+    L.gm_memcpy_d2h(pp.ctypes.data, out.p + 8 * r0, 8)
+    L.gm_memcpy_d2h(pp.ctypes.data + 8, out.p + 8 * r1, 8)
+    lo, hi = int(pp[0]), int(pp[1])
+    nz = hi - lo
+    sj, sx = ctypes.c_int64(0), ctypes.c_int32(0)
+    if nz > 0:
+        L.gm_reduce_scalar(ctypes.byref(sj), 1, 8, out.j + 8 * lo, nz, 0)
+        L.gm_reduce_scalar(ctypes.byref(sx), 1, 6, out.x + (0 if out.iso else 4 * lo), nz, out.iso)
+    return nz, sj.value - nz * col_base, sx.value
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -145,6 +418,14 @@ def main():
            "data": "synthetic", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "metric": "semiring mults/s for GrB_mxm",
            "unit": "mults/s"}
+    if args.workload in ("served-rmat20", "batched-4096"):
+        if args.workload == "served-rmat20":
+            out.update(bench_served(args, capi, W))
+        else:
+            out.update(bench_batched(args, capi, W))
+        out["device"] = dev
+        print(json.dumps(out))
+        return
 
     if args.workload == "saxpy3-er20":
         n = 1 << 20

@@ -441,6 +441,37 @@ def prof_get() -> dict:
     return {recs[t].name.decode(): (recs[t].launches, recs[t].total_ms) for t in range(min(n, 64))}
 
 
+class DeviceCsr:
+    """A host CSR copied into gm_malloc'd HBM; .view is its gm_csr struct."""
+
+    def __init__(self, m: Csr, type_code: int, iso=False):
+        L = lib()
+        self.bufs = []
+        self.m = m
+
+        def up(a):
+            a = np.ascontiguousarray(a)
+            d = L.gm_malloc(max(a.nbytes, 8))
+            if not d:
+                raise GrBError(-102, last_error())
+            if a.nbytes and L.gm_memcpy_h2d(d, a.ctypes.data, a.nbytes) != 0:
+                raise GrBError(-101, last_error())
+            self.bufs.append(d)
+            return d
+
+        p = up(np.asarray(m.p, dtype=np.int64))
+        j = up(np.asarray(m.j, dtype=np.int64)) if m.nnz > 0 else None
+        x = None
+        if m.x is not None:
+            x = up(m.x[:1] if iso else m.x)
+        self.view = GmCsr(m.nrows, m.ncols, p, j, x, 1 if iso else 0, type_code)
+
+    def free(self):
+        for d in self.bufs:
+            lib().gm_free(d)
+        self.bufs = []
+
+
 class Server:
     """Served frontier expansion on one resident graph (gm_server_*).
 

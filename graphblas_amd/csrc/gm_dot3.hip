@@ -94,6 +94,7 @@ struct Dot3Args
     unsigned long long *bin_cnt;  // 3 counters
     int64_t thread_max, team16_max, wave_max;
     RowBits bA, bB;               // bitmaps of the long rows of A and B
+    int32_t cols32;               // column indices fit 32 bits (LDS staging allowed)
 };
 
 // lane-aggregated append: returns the slot for lanes with pred set.
@@ -383,6 +384,106 @@ __device__ void dot_lane(const Dot3Args &g, const Pair &P, int lane, bool &e, ty
     }
 }
 
+// LDS words per team for staged pairs: both index lists (as 32-bit columns)
+// of a merge/search pair with a+b <= cap are copied to LDS by the team with
+// coalesced loads, then intersected there, so the merge walk and the binary
+// searches pay LDS latency instead of a chain of dependent global loads.
+template <int G> struct StageCap { static constexpr int value = G == 16 ? 512 : 2048; };
+
+// make the team's LDS stores visible to the team's later loads (the team is
+// part of one wavefront, so no workgroup barrier is needed)
+__device__ __forceinline__ void team_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Same partition of the work over lanes as dot_lane's merge/search cases, on
+// LDS copies sa[0..a) and sb[0..b) of the two lists.
+template <class S, int G>
+__device__ void dot_lane_staged(const Dot3Args &g, const Pair &P, int lane, uint32_t *sl, bool &e,
+                                typename S::Z &z)
+{
+    using Z = typename S::Z;
+    e = false;
+    z = S::identity();
+    const Z *Ax = (const Z *) g.Ax, *Bx = (const Z *) g.Bx;
+    const int64_t pa = P.pa, pb = P.pb;
+    const int a = (int) (P.pa_end - pa), b = (int) (P.pb_end - pb);
+    for (int t = lane; t < a; t += G) sl[t] = (uint32_t) g.Aj[pa + t];
+    for (int t = lane; t < b; t += G) sl[a + t] = (uint32_t) g.Bj[pb + t];
+    team_lds_sync();
+    const uint32_t *sa = sl, *sb = sl + a;
+    const int s = a < b ? a : b, l = a < b ? b : a;
+    const bool search = (int64_t) s * (int64_t) (ilog2_ceil(l + 1) + 1) < (int64_t) (a + b);
+    if (search)
+    {
+        const bool shortA = a <= b;
+        const uint32_t *Sj = shortA ? sa : sb;
+        const uint32_t *Lj = shortA ? sb : sa;
+        int lcur = 0;
+        const int chunk = (s + G - 1) / G;
+        const int q0 = lane * chunk;
+        const int q1 = s < q0 + chunk ? s : q0 + chunk;
+        for (int q = q0; q < q1; q++)
+        {
+            const uint32_t x = Sj[q];
+            int lo = lcur, hi = l;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi) >> 1;
+                if (Lj[mid] < x) lo = mid + 1; else hi = mid;
+            }
+            lcur = lo;
+            if (lcur >= l) break;
+            if (Lj[lcur] == x)
+            {
+                const int64_t qa = shortA ? pa + q : pa + lcur, qb = shortA ? pb + lcur : pb + q;
+                Z t = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), (int64_t) x);
+                if (!e) { z = t; e = true; } else z = S::upd(z, t);
+                if constexpr (S::is_any) break;
+                if (S::terminal(z)) break;
+                lcur++;
+            }
+        }
+    }
+    else
+    {
+        const int tot = a + b;
+        const int d0 = (tot * lane) / G, d1 = (tot * (lane + 1)) / G;
+        int lo = d0 > b ? d0 - b : 0, hi = d0 < a ? d0 : a;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (sa[mid] <= sb[d0 - mid - 1]) lo = mid + 1; else hi = mid;
+        }
+        int x = lo, y = d0 - lo;
+        for (int d = d0; d < d1; d++)
+        {
+            if (x < a && (y >= b || sa[x] <= sb[y]))
+            {
+                x++;
+            }
+            else
+            {
+                const uint32_t kb = sb[y];
+                if (x > 0 && sa[x - 1] == kb)
+                {
+                    Z t = S::mul(S::geta(Ax, pa + x - 1, g.a_iso), S::geta(Bx, pb + y, g.b_iso),
+                                 (int64_t) kb);
+                    if (!e) { z = t; e = true; } else z = S::upd(z, t);
+                    if constexpr (S::is_any) break;
+                    if (S::terminal(z)) break;
+                }
+                y++;
+            }
+        }
+    }
+    // the next pair of this team overwrites the lists
+    team_lds_sync();
+}
+
 // final existence rule and store of one entry
 template <class S>
 __device__ __forceinline__ void d3_store(const Dot3Args &g, int64_t p, bool e, typename S::Z z)
@@ -478,7 +579,10 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_team(Dot3Args g, int bin)
     const int64_t count = (int64_t) g.bin_cnt[bin];
     const uint64_t *list = g.bins + (int64_t) bin * g.mnz;
     constexpr int TPB = D3_NT / G;
+    constexpr int CAP = StageCap<G>::value;
+    __shared__ uint32_t s_lists[TPB][CAP];
     const int lane = threadIdx.x & (G - 1);
+    uint32_t *sl = s_lists[threadIdx.x / G];
     for (int64_t q = (int64_t) blockIdx.x * TPB + threadIdx.x / G; q < count;
          q += (int64_t) gridDim.x * TPB)
     {
@@ -489,7 +593,11 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_team(Dot3Args g, int bin)
         Z z;
         Pair P;
         (void) d3_pair(g, i, j, P);
-        dot_lane<S, G>(g, P, lane, e, z);
+        if (!P.empty && P.probe == 0 && g.cols32 &&
+            (P.pa_end - P.pa) + (P.pb_end - P.pb) <= CAP)
+            dot_lane_staged<S, G>(g, P, lane, sl, e, z);
+        else
+            dot_lane<S, G>(g, P, lane, e, z);
         team_reduce<S, G>(e, z);
         if (lane == 0) d3_store<S>(g, p, e, z);
     }
@@ -878,6 +986,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         g.Bp = B->p; g.Bj = B->j; g.Bx = B->x; g.b_iso = B->iso;
         g.W = W.ptr; g.live = live.as<uint8_t>(); g.bins = bins.as<uint64_t>();
         g.bin_cnt = cnt.as<unsigned long long>();
+        g.cols32 = (A->ncols < ((int64_t) 1 << 32)) ? 1 : 0;
         g.thread_max = (prm && prm->thread_max > 0) ? prm->thread_max : 32;
         g.team16_max = (prm && prm->team16_max > 0) ? prm->team16_max : 512;
         g.wave_max = (prm && prm->wave_max > 0) ? prm->wave_max : 4096;

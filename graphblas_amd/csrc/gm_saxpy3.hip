@@ -174,6 +174,42 @@ __device__ __forceinline__ T team_sum(T v)
 // max row lengths of A and B (decides whether every row is "small")
 //------------------------------------------------------------------------------
 
+// the same two maxima, with B's taken only over the rows that A references
+// (cheaper than a scan of B when A has far fewer rows than B, e.g. a few
+// frontier rows against a large graph); a lane per row, lanes stride its entries
+__global__ __launch_bounds__(SX_NT) void k_sx_maxlen_ref(const int64_t *__restrict__ Ap,
+                                                         const int64_t *__restrict__ Aj, int64_t m,
+                                                         const int64_t *__restrict__ Bp,
+                                                         unsigned long long *out)
+{
+    unsigned long long ma = 0, mb = 0;
+    for (int64_t i = (int64_t) blockIdx.x * SX_NT + threadIdx.x; i < m;
+         i += (int64_t) gridDim.x * SX_NT)
+    {
+        const int64_t p0 = Ap[i], p1 = Ap[i + 1];
+        const unsigned long long l = (unsigned long long) (p1 - p0);
+        ma = l > ma ? l : ma;
+        for (int64_t p = p0; p < p1; p++)
+        {
+            const int64_t k = Aj[p];
+            const unsigned long long lb = (unsigned long long) (Bp[k + 1] - Bp[k]);
+            mb = lb > mb ? lb : mb;
+        }
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        unsigned long long ua = __shfl_xor(ma, off, 64), ub = __shfl_xor(mb, off, 64);
+        ma = ua > ma ? ua : ma;
+        mb = ub > mb ? ub : mb;
+    }
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicMax(&out[0], ma);
+        atomicMax(&out[1], mb);
+    }
+}
+
 __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__ Ap, int64_t m,
                                                      const int64_t *__restrict__ Bp, int64_t kb,
                                                      unsigned long long *out)
@@ -1122,10 +1158,19 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     GM_HIP(hipGetLastError(), "k_sx_fill_u64");
     {
         ProfScope ps("sx_maxlen");
-        const int64_t lim = m > B->nrows ? m : B->nrows;
-        k_sx_maxlen<<<sx_grid(lim, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
-            A->p, m, B->p, B->nrows, g.cnt + 8);
-        GM_HIP(hipGetLastError(), "k_sx_maxlen");
+        if (m * 8 < B->nrows)
+        {
+            k_sx_maxlen_ref<<<sx_grid(m, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
+                A->p, A->j, m, B->p, g.cnt + 8);
+            GM_HIP(hipGetLastError(), "k_sx_maxlen_ref");
+        }
+        else
+        {
+            const int64_t lim = m > B->nrows ? m : B->nrows;
+            k_sx_maxlen<<<sx_grid(lim, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
+                A->p, m, B->p, B->nrows, g.cnt + 8);
+            GM_HIP(hipGetLastError(), "k_sx_maxlen");
+        }
     }
     GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
            "maxlen d2h");

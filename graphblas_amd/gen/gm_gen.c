@@ -84,12 +84,15 @@ static inline void rmat_edge(int scale, double a, double ab, double abc,
     *v = c;
 }
 
-// Symmetrised, deduplicated, loop-free RMAT graph in CSR.
-// Ap: n+1 entries; Aj: capacity >= 2*ef*n.  Returns nnz.
-int64_t gm_gen_rmat_sym(int scale, int64_t ef, double a, double b, double c,
-                        uint64_t seed, int64_t *Ap, int64_t *Aj)
+// Symmetrised, deduplicated, loop-free RMAT graph in CSR on n vertices (any
+// n >= 1): ef*n samples descend scale = ceil(log2 n) levels; a sample with
+// an endpoint >= n is dropped, as are self-loops.  For n = 2^scale nothing
+// is dropped but loops.  Ap: n+1 entries; Aj: capacity >= 2*ef*n.  Returns nnz.
+int64_t gm_gen_rmat_sym_n(int64_t n, int64_t ef, double a, double b, double c,
+                          uint64_t seed, int64_t *Ap, int64_t *Aj)
 {
-    const int64_t n = (int64_t) 1 << scale;
+    int scale = 0;
+    while (((int64_t) 1 << scale) < n) scale++;
     const int64_t ne = ef * n;
     const double ab = a + b, abc = a + b + c;
     int64_t *deg = (int64_t *) calloc((size_t) n + 1, sizeof(int64_t));
@@ -102,7 +105,7 @@ int64_t gm_gen_rmat_sym(int scale, int64_t ef, double a, double b, double c,
     {
         int64_t u, v;
         rmat_edge(scale, a, ab, abc, seed, (uint64_t) e, &u, &v);
-        if (u == v) continue;
+        if (u == v || u >= n || v >= n) continue;
         #pragma omp atomic
         deg[u]++;
         #pragma omp atomic
@@ -120,7 +123,7 @@ int64_t gm_gen_rmat_sym(int scale, int64_t ef, double a, double b, double c,
     {
         int64_t u, v, pu, pv;
         rmat_edge(scale, a, ab, abc, seed, (uint64_t) e, &u, &v);
-        if (u == v) continue;
+        if (u == v || u >= n || v >= n) continue;
         #pragma omp atomic capture
         pu = fill[u]++;
         #pragma omp atomic capture
@@ -158,6 +161,12 @@ int64_t gm_gen_rmat_sym(int scale, int64_t ef, double a, double b, double c,
     free(deg);
     free(fill);
     return pos;
+}
+
+int64_t gm_gen_rmat_sym(int scale, int64_t ef, double a, double b, double c,
+                        uint64_t seed, int64_t *Ap, int64_t *Aj)
+{
+    return gm_gen_rmat_sym_n((int64_t) 1 << scale, ef, a, b, c, seed, Ap, Aj);
 }
 
 // Strict lower (lower=1) or upper (lower=0) triangle of a CSR matrix with

@@ -27,6 +27,9 @@ def _lib():
         g = ctypes.CDLL(_GEN_SO)
         P = ctypes.c_void_p
         i64 = ctypes.c_int64
+        g.gm_gen_rmat_sym_n.restype = i64
+        g.gm_gen_rmat_sym_n.argtypes = [i64, i64, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, P, P]
         g.gm_gen_rmat_sym.restype = i64
         g.gm_gen_rmat_sym.argtypes = [ctypes.c_int, i64, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint64, P, P]
@@ -99,6 +102,18 @@ def rmat_sym(scale: int, ef: int, a=0.57, b=0.19, c=0.19, seed=1) -> Csr:
     return Csr(n, n, Ap, Aj)
 
 
+def rmat_sym_n(n: int, ef: int, a=0.57, b=0.19, c=0.19, seed=1) -> Csr:
+    """RMAT on any n (scale ceil(log2 n), samples with an endpoint >= n dropped),
+    symmetrised, deduplicated, loop-free (pattern only)."""
+    g = _lib()
+    Ap = np.empty(n + 1, dtype=np.int64)
+    Aj = np.empty(max(2 * ef * n, 1), dtype=np.int64)
+    nnz = g.gm_gen_rmat_sym_n(n, ef, a, b, c, seed, _ptr(Ap), _ptr(Aj))
+    if nnz < 0:
+        raise MemoryError("rmat generator out of memory")
+    return Csr(n, n, Ap, Aj[:nnz].copy())
+
+
 def triangle(A: Csr, lower: bool) -> Csr:
     """Strict lower (L) or upper (U) triangle of a matrix with sorted rows."""
     g = _lib()
@@ -144,3 +159,138 @@ def loguniform_int(seed: int, ctr: int, lo: int, hi: int) -> int:
     u = unif(seed, ctr)
     v = int(math.floor(math.exp(math.log(lo) + u * (math.log(hi + 1) - math.log(lo)))))
     return max(lo, min(hi, v))
+
+
+# ---------------------------------------------------------------------------
+# batches of uneven graphs
+# ---------------------------------------------------------------------------
+
+BATCH_SEMIRINGS = ("plus_pair_int32", "min_plus_int32")
+
+
+@dataclass
+class GraphSpec:
+    n: int
+    ef: int
+    semiring: str        # one of BATCH_SEMIRINGS
+    seed: int            # structure
+    wseed: int           # int32 edge weights in [1, 100]
+
+
+def batch_specs(ngraph: int, seed: int = 1, n_lo=512, n_hi=65536, ef_lo=4, ef_hi=32):
+    """ngraph RMAT requests: n log-uniform in [n_lo, n_hi], edge factor
+    uniform in [ef_lo, ef_hi], semiring plus_pair int32 or min_plus int32 with
+    probability 1/2 each; per-graph seeds from counter draws."""
+    out = []
+    for g in range(ngraph):
+        n = loguniform_int(seed, 4 * g, n_lo, n_hi)
+        ef = ef_lo + min(ef_hi - ef_lo, int(unif(seed, 4 * g + 1) * (ef_hi - ef_lo + 1)))
+        sr = BATCH_SEMIRINGS[0] if unif(seed, 4 * g + 2) < 0.5 else BATCH_SEMIRINGS[1]
+        gs = rand64(seed, 4 * g + 3)
+        out.append(GraphSpec(n, ef, sr, gs, gs ^ 0x5DEECE66D))
+    return out
+
+
+def batch_graph_host(spec: GraphSpec) -> Csr:
+    """One batch graph built on the host (the device generator's reference)."""
+    A = rmat_sym_n(spec.n, spec.ef, seed=spec.seed)
+    A.x = sym_weights_i32(A, seed=spec.wseed)
+    return A
+
+
+_gendev = None
+
+
+def _devlib():
+    global _gendev
+    if _gendev is None:
+        from . import capi
+        capi.lib()          # the kernel library first (the generator links it)
+        so = os.path.join(_HERE, "lib", "libgm_gendev.so")
+        if not os.path.exists(so):
+            raise RuntimeError(f"{so} is not built; run __graft_entry__.build()")
+        d = ctypes.CDLL(so)
+        P = ctypes.c_void_p
+        d.gm_gen_rmat_batch_dev.restype = ctypes.c_int
+        d.gm_gen_rmat_batch_dev.argtypes = [ctypes.c_int64, P, P, P, P, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                            ctypes.c_int32, P, P, ctypes.POINTER(P),
+                                            ctypes.POINTER(P), ctypes.POINTER(P)]
+        d.gm_server_drive.restype = ctypes.c_int
+        d.gm_server_drive.argtypes = [P, ctypes.c_int64, P, P, P, P, P]
+        _gendev = d
+    return _gendev
+
+
+def server_drive(server, reqs, arrive_us=None):
+    """Open-loop run of reqs (lists of sources) through a capi.Server: request
+    r is submitted arrive_us[r] microseconds after the start (all at once when
+    None) by the C++ load generator.  Returns (results, wall seconds)."""
+    from . import capi
+    d = _devlib()
+    rp = np.zeros(len(reqs) + 1, dtype=np.int64)
+    rp[1:] = np.cumsum([len(r) for r in reqs])
+    src = np.ascontiguousarray(np.concatenate(reqs) if reqs else np.zeros(1), dtype=np.int64)
+    arr = None if arrive_us is None else np.ascontiguousarray(arrive_us, dtype=np.float64)
+    out = (capi.GmServerResult * max(len(reqs), 1))()
+    wall = ctypes.c_double()
+    rc = d.gm_server_drive(server.h, len(reqs), _ptr(rp), _ptr(src), _ptr(arr), out,
+                           ctypes.byref(wall))
+    if rc != 0:
+        raise RuntimeError(f"gm_server_drive failed ({rc}): {capi.last_error()}")
+    return [out[r] for r in range(len(reqs))], wall.value * 1e-6
+
+
+class DeviceBatch:
+    """Batch graphs generated in HBM (same graphs as batch_graph_host).
+
+    views[g] is graph g's gm_csr (int32 weights); free() releases the arrays."""
+
+    def __init__(self, specs, a=0.57, b=0.19, c=0.19):
+        from . import capi
+        d = _devlib()
+        ng = len(specs)
+        n = np.array([s.n for s in specs], dtype=np.int64)
+        ef = np.array([s.ef for s in specs], dtype=np.int64)
+        seed = np.array([s.seed for s in specs], dtype=np.uint64)
+        wseed = np.array([s.wseed for s in specs], dtype=np.uint64)
+        self.pbase = np.zeros(ng + 1, dtype=np.int64)
+        self.nzbase = np.zeros(ng + 1, dtype=np.int64)
+        P, J, X = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        rc = d.gm_gen_rmat_batch_dev(ng, _ptr(n), _ptr(ef), _ptr(seed), _ptr(wseed), a, b, c,
+                                     1, 100, _ptr(self.pbase), _ptr(self.nzbase),
+                                     ctypes.byref(P), ctypes.byref(J), ctypes.byref(X))
+        if rc != 0:
+            raise RuntimeError(f"gm_gen_rmat_batch_dev failed ({rc})")
+        self.P, self.J, self.X = P.value, J.value, X.value
+        self.specs = specs
+        self.views = []
+        for g, s in enumerate(specs):
+            self.views.append(capi.GmCsr(s.n, s.n, self.P + 8 * int(self.pbase[g]),
+                                         self.J + 8 * int(self.nzbase[g]),
+                                         self.X + 4 * int(self.nzbase[g]), 0, 6))
+
+    def nnz(self, g: int) -> int:
+        return int(self.nzbase[g + 1] - self.nzbase[g])
+
+    def download(self, g: int) -> Csr:
+        from . import capi
+        L = capi.lib()
+        s = self.specs[g]
+        nz = self.nnz(g)
+        p = np.empty(s.n + 1, dtype=np.int64)
+        j = np.empty(max(nz, 1), dtype=np.int64)
+        x = np.empty(max(nz, 1), dtype=np.int32)
+        L.gm_memcpy_d2h(p.ctypes.data, self.P + 8 * int(self.pbase[g]), p.nbytes)
+        if nz:
+            L.gm_memcpy_d2h(j.ctypes.data, self.J + 8 * int(self.nzbase[g]), 8 * nz)
+            L.gm_memcpy_d2h(x.ctypes.data, self.X + 4 * int(self.nzbase[g]), 4 * nz)
+        return Csr(s.n, s.n, p, j[:nz], x[:nz])
+
+    def free(self):
+        from . import capi
+        L = capi.lib()
+        for ptr in (self.P, self.J, self.X):
+            if ptr:
+                L.gm_free(ptr)
+        self.P = self.J = self.X = None

@@ -86,34 +86,7 @@ def assert_csr_equal(got: Csr, ref: Csr, fp_tol=None, what=""):
 # direct C-ABI (gm_*) calls on device copies of host CSR arrays
 # ---------------------------------------------------------------------------
 
-class DevCsr:
-    """Host CSR copied into gm_malloc'd HBM; .view is the gm_csr struct."""
-
-    def __init__(self, m: Csr, type_code: int, iso=False):
-        L = capi.lib()
-        self.bufs = []
-        self.m = m
-
-        def up(a):
-            a = np.ascontiguousarray(a)
-            d = L.gm_malloc(max(a.nbytes, 8))
-            assert d, capi.last_error()
-            if a.nbytes:
-                assert L.gm_memcpy_h2d(d, a.ctypes.data, a.nbytes) == 0, capi.last_error()
-            self.bufs.append(d)
-            return d
-
-        p = up(np.asarray(m.p, dtype=np.int64))
-        j = up(np.asarray(m.j, dtype=np.int64)) if m.nnz > 0 else None
-        x = None
-        if m.x is not None:
-            x = up(m.x[:1] if iso else m.x)
-        self.view = capi.GmCsr(m.nrows, m.ncols, p, j, x, 1 if iso else 0, type_code)
-
-    def free(self):
-        for d in self.bufs:
-            capi.lib().gm_free(d)
-        self.bufs = []
+DevCsr = capi.DeviceCsr
 
 
 def download(out: capi.GmCsrOut, zdtype) -> Csr:

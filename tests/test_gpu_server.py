@@ -112,3 +112,22 @@ def test_server_plus_times_fp64_and_bad_source(grb):
     finally:
         srv.close()
         dG.free()
+
+
+def test_server_drive_poisson(grb):
+    """The C++ load generator submits on a Poisson schedule; every request
+    completes, in its own batch or in a mixed one."""
+    G = W.rmat_sym(11, 16, seed=9)
+    dG = DevCsr(G, 1)
+    srv = capi.Server(dG.view, ANY_SECONDI, workers=2)
+    try:
+        reqs = [s for s, _ in _requests(G.nrows, 200, seed=8, vis_frac=0.0)]
+        arrive = np.cumsum(np.random.default_rng(2).exponential(50.0, len(reqs)))
+        res, wall = W.server_drive(srv, reqs, arrive)
+        assert all(r.status == 0 for r in res)
+        assert wall * 1e6 >= arrive[-1]
+        assert sum(r.k for r in res) == sum(len(s) for s in reqs)
+        assert srv.stats()["completed"] == len(reqs)
+    finally:
+        srv.close()
+        dG.free()
