@@ -27,6 +27,8 @@
 //          of the surviving (j, value) pairs, and the new row pointers of all
 //          rows that start in the tile.  The result has no zombies.
 
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 #include "gm_internal.h"
 #include "gm_block.h"
 
@@ -95,6 +97,8 @@ struct Dot3Args
     int64_t thread_max, team16_max, wave_max;
     RowBits bA, bB;               // bitmaps of the long rows of A and B
     int32_t cols32;               // column indices fit 32 bits (LDS staging allowed)
+    uint64_t *bkeys;              // NULL, or 3 x mnz grouping keys of the binned entries
+    int64_t key_nA;               // probe-2 keys are key_nA + j
 };
 
 // lane-aggregated append: returns the slot for lanes with pred set.
@@ -528,6 +532,7 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
             const bool valid = p < t1;
             int bin = -1;
             int64_t i = 0;
+            uint64_t key = 0;
             if (valid)
             {
                 int64_t lo = r0, hi = r1;
@@ -548,6 +553,8 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                     {
                         if (w <= g.thread_max) e = dot_serial<S>(g, P, z);
                         else bin = (w <= g.team16_max) ? 0 : (w <= g.wave_max) ? 1 : 2;
+                        // entries probing the same long row's index sort together
+                        key = (P.probe == 2) ? (uint64_t) (g.key_nA + j) : (uint64_t) i;
                     }
                 }
                 if (bin < 0)
@@ -561,7 +568,12 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
             {
                 const bool take = (bin == b);
                 const unsigned long long slot = wave_append(take, &g.bin_cnt[b]);
-                if (take) g.bins[(int64_t) b * g.mnz + (int64_t) slot] = ((uint64_t) i << 32) | (uint64_t) p;
+                if (take)
+                {
+                    g.bins[(int64_t) b * g.mnz + (int64_t) slot] = ((uint64_t) i << 32) | (uint64_t) p;
+                    // keys only for the two team bins (bkeys holds 2 x mnz)
+                    if (g.bkeys && b < 2) g.bkeys[(int64_t) b * g.mnz + (int64_t) slot] = key;
+                }
             }
         }
         __syncthreads();
@@ -572,34 +584,74 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
 // team kernels: G lanes per deferred entry (G = 16 or 64)
 //------------------------------------------------------------------------------
 
+// one deferred entry w = (i << 32 | p), computed by a team of G lanes
+template <class S, int G>
+__device__ __forceinline__ void team_task(const Dot3Args &g, uint64_t w, int lane, uint32_t *sl)
+{
+    using Z = typename S::Z;
+    constexpr int CAP = StageCap<G>::value;
+    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+    const int64_t j = g.Mj[p];
+    bool e;
+    Z z;
+    Pair P;
+    (void) d3_pair(g, i, j, P);
+    if (!P.empty && P.probe == 0 && g.cols32 &&
+        (P.pa_end - P.pa) + (P.pb_end - P.pb) <= CAP)
+        dot_lane_staged<S, G>(g, P, lane, sl, e, z);
+    else
+        dot_lane<S, G>(g, P, lane, e, z);
+    team_reduce<S, G>(e, z);
+    if (lane == 0) d3_store<S>(g, p, e, z);
+}
+
+// the bin's entries in their appended order, strided over the grid
 template <class S, int G>
 __global__ __launch_bounds__(D3_NT) void k_dot3_team(Dot3Args g, int bin)
 {
-    using Z = typename S::Z;
     const int64_t count = (int64_t) g.bin_cnt[bin];
     const uint64_t *list = g.bins + (int64_t) bin * g.mnz;
     constexpr int TPB = D3_NT / G;
-    constexpr int CAP = StageCap<G>::value;
-    __shared__ uint32_t s_lists[TPB][CAP];
+    __shared__ uint32_t s_lists[TPB][StageCap<G>::value];
     const int lane = threadIdx.x & (G - 1);
     uint32_t *sl = s_lists[threadIdx.x / G];
     for (int64_t q = (int64_t) blockIdx.x * TPB + threadIdx.x / G; q < count;
          q += (int64_t) gridDim.x * TPB)
+        team_task<S, G>(g, list[q], lane, sl);
+}
+
+// The bin's entries sorted by the row whose probe index they read.  The list
+// is cut into 8 regions, one per XCD (workgroups are dispatched to the 8 XCDs
+// round-robin, so workgroup b runs on XCD b mod 8): an XCD's workgroups take
+// chunks of its own region from a counter, so the entries that probe one
+// long row run on one XCD and find that row's index in its L2.  A workgroup
+// whose region is used up moves on to the next regions.
+template <class S, int G>
+__global__ __launch_bounds__(D3_NT) void k_dot3_team_grp(Dot3Args g, const uint64_t *list,
+                                                         int64_t count, unsigned long long *ctr)
+{
+    constexpr int TPB = D3_NT / G;
+    constexpr int CH = TPB * 4;
+    __shared__ uint32_t s_lists[TPB][StageCap<G>::value];
+    __shared__ int64_t s_start;
+    const int lane = threadIdx.x & (G - 1);
+    const int team = threadIdx.x / G;
+    uint32_t *sl = s_lists[team];
+    const int x0 = (int) (blockIdx.x & 7);
+    for (int k = 0; k < 8; k++)
     {
-        const uint64_t w = list[q];
-        const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
-        const int64_t j = g.Mj[p];
-        bool e;
-        Z z;
-        Pair P;
-        (void) d3_pair(g, i, j, P);
-        if (!P.empty && P.probe == 0 && g.cols32 &&
-            (P.pa_end - P.pa) + (P.pb_end - P.pb) <= CAP)
-            dot_lane_staged<S, G>(g, P, lane, sl, e, z);
-        else
-            dot_lane<S, G>(g, P, lane, e, z);
-        team_reduce<S, G>(e, z);
-        if (lane == 0) d3_store<S>(g, p, e, z);
+        const int r = (x0 + k) & 7;
+        const int64_t lo = count * r / 8, hi = count * (r + 1) / 8;
+        while (true)
+        {
+            if (threadIdx.x == 0) s_start = lo + (int64_t) atomicAdd(&ctr[r], 1ull) * CH;
+            __syncthreads();
+            const int64_t start = s_start;
+            __syncthreads();
+            if (start >= hi) break;
+            const int64_t end = start + CH < hi ? start + CH : hi;
+            for (int64_t q = start + team; q < end; q += TPB) team_task<S, G>(g, list[q], lane, sl);
+        }
     }
 }
 
@@ -946,6 +998,68 @@ static int dot3_bitmaps(Dot3Args &g, const gm_csr *A, const gm_csr *B, bool need
     return GM_SUCCESS;
 }
 
+// Sort the team16 and wave bins by their grouping keys (radix sort of
+// (key, entry) pairs) and run them with the XCD-regioned team kernel.  The
+// bins' keys must have been written by phase1 (g.bkeys != NULL).
+template <class S>
+static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
+{
+    hipStream_t st = stream();
+    unsigned long long *hc = (unsigned long long *) pinned_scratch(4 * sizeof(unsigned long long));
+    if (!hc) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+           "bin counts");
+    GM_HIP(hipStreamSynchronize(st), "bin counts sync");
+    const int64_t cnt[2] = {(int64_t) hc[0], (int64_t) hc[1]};
+    const int64_t big = cnt[0] > cnt[1] ? cnt[0] : cnt[1];
+    if (big == 0) return GM_SUCCESS;
+    int end_bit = 1;
+    while (end_bit < 64 && ((int64_t) 1 << end_bit) <= nkeys) end_bit++;
+    DevBuf tk, tv, tmp, ctr;
+    if (!tk.alloc(sizeof(uint64_t) * big) || !tv.alloc(sizeof(uint64_t) * big) ||
+        !ctr.alloc(16 * sizeof(unsigned long long)))
+        return set_error(GM_OUT_OF_MEMORY, "dot3: grouping buffers");
+    size_t tbytes = 0;
+    for (int b = 0; b < 2; b++)
+    {
+        size_t tb = 0;
+        GM_HIP(rocprim::radix_sort_pairs(nullptr, tb, (const uint64_t *) g.bkeys, tk.as<uint64_t>(),
+                                         (const uint64_t *) g.bins, tv.as<uint64_t>(), (size_t) cnt[b],
+                                         0, (unsigned) end_bit, st), "grouping sort size");
+        if (tb > tbytes) tbytes = tb;
+    }
+    if (!tmp.alloc(tbytes)) return set_error(GM_OUT_OF_MEMORY, "dot3: grouping sort");
+    k_zero_u64<<<1, 64, 0, st>>>(ctr.as<unsigned long long>(), 16);
+    for (int b = 0; b < 2; b++)
+    {
+        if (cnt[b] == 0) continue;
+        {
+            ProfScope ps("dot3_group_sort");
+            size_t tb = tbytes;
+            GM_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, (const uint64_t *) g.bkeys + (int64_t) b * g.mnz,
+                                             tk.as<uint64_t>(),
+                                             (const uint64_t *) g.bins + (int64_t) b * g.mnz,
+                                             tv.as<uint64_t>(), (size_t) cnt[b], 0, (unsigned) end_bit, st),
+                   "grouping sort");
+        }
+        if (b == 0)
+        {
+            ProfScope ps("dot3_team16");
+            k_dot3_team_grp<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                g, tv.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
+            GM_HIP(hipGetLastError(), "k_dot3_team_grp16");
+        }
+        else
+        {
+            ProfScope ps("dot3_wave");
+            k_dot3_team_grp<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                g, tv.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
+            GM_HIP(hipGetLastError(), "k_dot3_team_grp64");
+        }
+    }
+    return GM_SUCCESS;
+}
+
 template <class S>
 static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
                     const gm_csr *B, const gm_dot3_params *prm)
@@ -987,6 +1101,19 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         g.W = W.ptr; g.live = live.as<uint8_t>(); g.bins = bins.as<uint64_t>();
         g.bin_cnt = cnt.as<unsigned long long>();
         g.cols32 = (A->ncols < ((int64_t) 1 << 32)) ? 1 : 0;
+        g.bkeys = nullptr;
+        g.key_nA = A->nrows;
+        // group deferred entries by probed row: explicit, or by default for
+        // masks large enough that the sort pays for itself
+        const int64_t order = prm ? prm->order : 0;
+        const bool grouped = (order == 2) || (order == 0 && Mnz >= ((int64_t) 1 << 20));
+        DevBuf bkeys;
+        if (grouped)
+        {
+            if (!bkeys.alloc(sizeof(uint64_t) * 2 * Mnz))
+                return set_error(GM_OUT_OF_MEMORY, "dot3: grouping keys");
+            g.bkeys = bkeys.as<uint64_t>();
+        }
         g.thread_max = (prm && prm->thread_max > 0) ? prm->thread_max : 32;
         g.team16_max = (prm && prm->team16_max > 0) ? prm->team16_max : 512;
         g.wave_max = (prm && prm->wave_max > 0) ? prm->wave_max : 4096;
@@ -1005,15 +1132,22 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
             k_dot3_phase1<S><<<grid_for(Mnz, D3_P1_TILE, cap), D3_NT, 0, st>>>(g);
             GM_HIP(hipGetLastError(), "k_dot3_phase1");
         }
+        if (grouped)
         {
-            ProfScope ps("dot3_team16");
-            k_dot3_team<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 0);
-            GM_HIP(hipGetLastError(), "k_dot3_team16");
+            GM_TRY(dot3_grouped_teams<S>(g, A->nrows + B->nrows));
         }
+        else
         {
-            ProfScope ps("dot3_wave");
-            k_dot3_team<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 1);
-            GM_HIP(hipGetLastError(), "k_dot3_wave");
+            {
+                ProfScope ps("dot3_team16");
+                k_dot3_team<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 0);
+                GM_HIP(hipGetLastError(), "k_dot3_team16");
+            }
+            {
+                ProfScope ps("dot3_wave");
+                k_dot3_team<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g, 1);
+                GM_HIP(hipGetLastError(), "k_dot3_wave");
+            }
         }
         {
             ProfScope ps("dot3_block");

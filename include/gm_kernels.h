@@ -110,6 +110,9 @@ typedef struct
                           // larger: one 256-thread workgroup per entry
     int64_t bitmap_min_len;  // rows at least this long may get a bitmap (256);
                              // < 0: no bitmaps (merge / search only)
+    int64_t order;        // order of the team16 / wave entries: 0 auto (grouped
+                          // when nnz(M) >= 2^20), 1 mask order, 2 grouped by
+                          // the row whose probe index they read
 } gm_dot3_params;
 
 // saxpy3 tuning knobs (0 = default).  f = flops of a row; t = its table

@@ -140,3 +140,33 @@ def test_dot3_bitmaps_distinct_operands(grb, handle):
     ref = O.dot3(name, M, A, B)
     got = gm_dot3(handle, M, A, B, params=(0, 0, 0, 8))
     assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what="bitmaps")
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("params", [(1, 2, 100000, -1), (1, 64, 100000, 8), (1, 16, 100000, 1),
+                                    (0, 0, 0, 0)])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_ANY_SECONDI_INT64"])
+def test_dot3_entry_order(grb, order, params, handle):
+    """Team entries in mask order (1) and grouped by probed row over the XCD
+    regions (2) give the oracle's result, with and without probe indexes."""
+    print(f"[dot3] order={order} {params} {handle}", flush=True)
+    rng = np.random.default_rng(21)
+    vt = _vals(handle)
+    A = power_law_csr(700, 25, rng, values=vt)
+    B = power_law_csr(700, 25, rng, values=vt)
+    M = power_law_csr(700, 20, rng, values="int64")
+    name, _ = SR[handle]
+    ref = O.dot3(name, M, A, B)
+    got = gm_dot3(handle, M, A, B, params=tuple(params) + (order,))
+    assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=f"order {order}")
+
+
+def test_dot3_grouped_tricount_default(grb):
+    """A mask of more than 2^20 entries takes the grouped order by default."""
+    A, L, U = W.tricount_inputs(17, 16, seed=2)
+    assert L.nnz >= (1 << 20)
+    print(f"[dot3] grouped default tricount scale 17 nnz(L)={L.nnz}", flush=True)
+    ref = O.dot3("plus_pair_int64", L, L, U)
+    got = gm_dot3("GxB_PLUS_PAIR_INT64", L, L, U, abtype=1, mtype=1)
+    assert_csr_equal(got, ref, what="tricount scale 17")
