@@ -233,6 +233,13 @@ def lib():
                             ctypes.c_int),
         "gm_batch_split": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsrOut), P, P, I64],
                            ctypes.c_int),
+        "gm_transpose": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr)], ctypes.c_int),
+        "gm_accum_mask": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
+                           ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr),
+                           I32], ctypes.c_int),
+        "gm_dot3_set_defaults": ([ctypes.POINTER(GmDot3Params)], ctypes.c_int),
+        "gm_saxpy3_set_defaults": ([ctypes.POINTER(GmSaxpy3Params)], ctypes.c_int),
+        "gm_load_tuning": ([ctypes.c_char_p], ctypes.c_int),
         "gm_server_create": ([ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
                               ctypes.POINTER(GmServerConfig)], P),
         "gm_server_submit": ([P, P, I64, P, P], I64),
@@ -391,8 +398,8 @@ def mxm(C: Matrix, M: Optional[Matrix], semiring: str, A: Matrix, B: Matrix,
                          A.h, B.h, desc(descriptor))
 
 
-def mxm_checked(C, M, semiring, A, B, descriptor=None):
-    _check(mxm(C, M, semiring, A, B, descriptor), "GrB_mxm", C.h)
+def mxm_checked(C, M, semiring, A, B, descriptor=None, accum=None):
+    _check(mxm(C, M, semiring, A, B, descriptor, handle(accum) if accum else None), "GrB_mxm", C.h)
 
 
 def reduce_sum_int64(A: Matrix) -> int:
@@ -426,6 +433,18 @@ def device_info() -> dict:
     return {"name": info.name.decode(), "arch": info.arch.decode(), "cu_count": info.cu_count,
             "lds_per_block": info.lds_per_block, "hbm_bytes": info.hbm_bytes,
             "hbm_peak_gbs_hip_attrs": info.hbm_peak_gbs}
+
+
+def load_tuning(path: str):
+    """Process-wide bin cutoffs from an autotuner file (graphblas_amd/tune.py)."""
+    rc = lib().gm_load_tuning(path.encode())
+    if rc != 0:
+        raise GrBError(rc, last_error())
+
+
+def reset_tuning():
+    lib().gm_dot3_set_defaults(None)
+    lib().gm_saxpy3_set_defaults(None)
 
 
 def prof_enable(on=True):

@@ -52,6 +52,17 @@ int scan_exclusive_i64(const int64_t *d_in, int64_t *d_out, int64_t n);
 
 size_t type_size(int32_t type);
 
+// process-wide tuned values (gm_dot3_set_defaults / gm_saxpy3_set_defaults /
+// gm_load_tuning); a 0 field means "built-in value"
+gm_dot3_params dot3_tuned();
+gm_saxpy3_params saxpy3_tuned();
+
+// explicit value if set (> 0), else the tuned one if set, else the built-in
+inline int64_t pick_param(int64_t explicit_v, int64_t tuned_v, int64_t builtin)
+{
+    return explicit_v > 0 ? explicit_v : (tuned_v > 0 ? tuned_v : builtin);
+}
+
 } // namespace gm
 
 #define GM_TRY(call)                     \

@@ -288,9 +288,83 @@ using namespace gm;
 // C ABI
 //------------------------------------------------------------------------------
 
+namespace gm
+{
+static gm_dot3_params g_d3_tuned = {0, 0, 0, 0, 0};
+static gm_saxpy3_params g_sx_tuned = {0, 0, 0, 0};
+
+gm_dot3_params dot3_tuned()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_d3_tuned;
+}
+
+gm_saxpy3_params saxpy3_tuned()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_sx_tuned;
+}
+} // namespace gm
+
 extern "C" {
 
-int gm_init(void)
+int gm_dot3_set_defaults(const gm_dot3_params *p)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_d3_tuned = p ? *p : gm_dot3_params{0, 0, 0, 0, 0};
+    return GM_SUCCESS;
+}
+
+int gm_saxpy3_set_defaults(const gm_saxpy3_params *p)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_sx_tuned = p ? *p : gm_saxpy3_params{0, 0, 0, 0};
+    return GM_SUCCESS;
+}
+
+int gm_load_tuning(const char *path)
+{
+    if (!path) return set_error(GM_INVALID_VALUE, "gm_load_tuning: NULL path");
+    FILE *f = fopen(path, "r");
+    if (!f) return set_error(GM_INVALID_VALUE, "gm_load_tuning: cannot open %s", path);
+    gm_dot3_params d = dot3_tuned();
+    gm_saxpy3_params x = saxpy3_tuned();
+    char line[256];
+    int rc = GM_SUCCESS;
+    while (fgets(line, sizeof(line), f))
+    {
+        char key[128];
+        long long v;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        if (sscanf(line, "%127s %lld", key, &v) != 2)
+        {
+            rc = set_error(GM_INVALID_VALUE, "gm_load_tuning: bad line in %s: %s", path, line);
+            break;
+        }
+        const std::string k(key);
+        if (k == "dot3.thread_max") d.thread_max = v;
+        else if (k == "dot3.team16_max") d.team16_max = v;
+        else if (k == "dot3.wave_max") d.wave_max = v;
+        else if (k == "dot3.bitmap_min_len") d.bitmap_min_len = v;
+        else if (k == "dot3.order") d.order = v;
+        else if (k == "saxpy3.wave_small_max") x.wave_small_max = v;
+        else if (k == "saxpy3.wave_large_max") x.wave_large_max = v;
+        else if (k == "saxpy3.block_max") x.block_max = v;
+        else if (k == "saxpy3.mode") x.mode = v;
+        else
+        {
+            rc = set_error(GM_INVALID_VALUE, "gm_load_tuning: unknown key %s in %s", key, path);
+            break;
+        }
+    }
+    fclose(f);
+    if (rc != GM_SUCCESS) return rc;
+    gm_dot3_set_defaults(&d);
+    gm_saxpy3_set_defaults(&x);
+    return GM_SUCCESS;
+}
+
+static int gm_init_device(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_init_rc == GM_SUCCESS) return GM_SUCCESS;
@@ -326,6 +400,22 @@ int gm_init(void)
     t_stream = g_stream;
     g_init_rc = GM_SUCCESS;
     return GM_SUCCESS;
+}
+
+// device set-up, then (once per process) the tuning file named by the
+// environment variable GM_TUNING_FILE, if set
+int gm_init(void)
+{
+    const int rc = gm_init_device();
+    if (rc == GM_SUCCESS)
+    {
+        static std::once_flag once;
+        std::call_once(once, [] {
+            const char *path = getenv("GM_TUNING_FILE");
+            if (path && *path) (void) gm_load_tuning(path);
+        });
+    }
+    return rc;
 }
 
 void gm_finalize(void)

@@ -733,7 +733,8 @@ __global__ void k_sx_iso_one(void *x, int size)
 // only waits on tiles already running; a bounded spin turns a lost
 // predecessor into an error flag instead of a hang.
 
-constexpr int FUSED_ROWS = SX_NT / TG;
+constexpr int FUSED_WAVES = SX_NT / 64;       // independent wavefronts per workgroup
+constexpr int FUSED_ROWS = 64 / TG;           // rows of a tile: one wavefront's 8 teams
 constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
@@ -746,29 +747,34 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                                                     int *err)
 {
     using Z = typename S::Z;
+    constexpr int NTEAM = SX_NT / TG;
     // per-team rows padded by one element so the 8 teams of a wavefront
     // start on different LDS banks
-    __shared__ uint64_t s_key[FUSED_ROWS][FMAX + 1];  // sort keys, then output columns (int32)
-    __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
-    __shared__ int64_t s_pre[FUSED_ROWS][TG + 1], s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
-    __shared__ Z s_av[FUSED_ROWS][TG + 1];
-    __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
-    __shared__ int s_big[FUSED_ROWS];
-    __shared__ int64_t s_tile, s_prefix;
+    __shared__ uint64_t s_key[NTEAM][FMAX + 1];  // sort keys, then output columns (int32)
+    __shared__ Z s_val[NTEAM][FMAX + 1];         // flop values, then output values
+    __shared__ int64_t s_pre[NTEAM][TG + 1], s_k[NTEAM][TG + 1], s_bs[NTEAM][TG + 1];
+    __shared__ Z s_av[NTEAM][TG + 1];
+    __shared__ int64_t s_cnt[NTEAM];
+    __shared__ int64_t s_off[FUSED_WAVES][FUSED_ROWS + 1];
+    __shared__ int s_big[NTEAM];
 
+    // every wavefront works on its own tiles: no workgroup barriers
     const int team = threadIdx.x / TG, sub = threadIdx.x & (TG - 1);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tw = team & (FUSED_ROWS - 1);       // team within the wavefront
     uint64_t *key = s_key[team];
     Z *val = s_val[team];
     int32_t *outj = (int32_t *) s_key[team];
+    int64_t *off = s_off[wv];
     unsigned long long *ticket = status + ntiles;
 
     while (true)
     {
-        if (threadIdx.x == 0) s_tile = (int64_t) atomicAdd(ticket, 1ull);
-        __syncthreads();
-        const int64_t tile = s_tile;
+        unsigned long long tk = 0;
+        if (lane == 0) tk = atomicAdd(ticket, 1ull);
+        const int64_t tile = (int64_t) __shfl(tk, 0, 64);
         if (tile >= ntiles) break;
-        const int64_t i = tile * FUSED_ROWS + team;
+        const int64_t i = tile * FUSED_ROWS + tw;
 
         #pragma unroll
         for (int u = 0; u < FMAX / TG; u++) key[sub * (FMAX / TG) + u] = ~0ull;
@@ -933,105 +939,101 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
             s_cnt[team] = cnt;
             s_big[team] = big;
         }
-        __syncthreads();
-        if (threadIdx.x < 64)
+        wave_sync();
+        // tile offsets of the wavefront's rows, then a look-back over up to
+        // LB_WIN x 64 predecessors per step (lane l reads tile - 1 - u*64 - l)
+        const int tbase = wv * FUSED_ROWS;
+        int64_t run = 0;
+        #pragma unroll
+        for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[tbase + t];
+        if (lane == 0)
         {
-            // wavefront 0: tile offsets, then a look-back over up to 64
-            // predecessors per step (lane l reads tile - 1 - l)
-            const int lane = threadIdx.x;
-            int64_t run = 0;
-            for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[t];
-            if (lane == 0)
-            {
-                int64_t r2 = 0;
-                for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = r2; r2 += s_cnt[t]; }
-                s_off[FUSED_ROWS] = r2;
-                __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            int64_t prefix = 0;
-            if (tile > 0)
-            {
-                // window of LB_WIN*64 predecessors, position u*64 + lane =
-                // distance - 1; all loads are independent (relaxed: the
-                // counts travel inside the status words themselves)
-                int64_t end = tile - 1;
-                int spins = 0;
-                while (true)
-                {
-                    unsigned long long w[LB_WIN];
-                    #pragma unroll
-                    for (int u = 0; u < LB_WIN; u++)
-                    {
-                        const int64_t idx = end - (int64_t) u * 64 - lane;
-                        w[u] = (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT)
-                                          : LB_PRE;
-                    }
-                    int firstp = LB_WIN * 64;
-                    bool zero_before = false;
-                    #pragma unroll
-                    for (int u = 0; u < LB_WIN; u++)
-                    {
-                        const unsigned long long flag = w[u] & ~LB_MASK;
-                        const unsigned long long bp = __ballot(flag == LB_PRE);
-                        const unsigned long long b0 = __ballot(flag == 0);
-                        if (firstp == LB_WIN * 64)
-                        {
-                            const int fp = bp ? (__ffsll((long long) bp) - 1) : 64;
-                            const unsigned long long upto =
-                                (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
-                            if (b0 & upto) zero_before = true;
-                            if (fp < 64) firstp = u * 64 + fp;
-                        }
-                    }
-                    if (zero_before)
-                    {
-                        // a predecessor has not published yet
-                        if (++spins > LB_SPIN_LIMIT)
-                        {
-                            if (lane == 0) atomicExch(err, 1);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    int64_t v = 0;
-                    #pragma unroll
-                    for (int u = 0; u < LB_WIN; u++)
-                        if (u * 64 + lane <= firstp) v += (int64_t) (w[u] & LB_MASK);
-                    #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                    prefix += v;
-                    if (firstp < LB_WIN * 64) break;
-                    end -= LB_WIN * 64;
-                }
-                if (lane == 0)
-                    __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (lane == 0) s_prefix = prefix;
+            int64_t r2 = 0;
+            for (int t = 0; t < FUSED_ROWS; t++) { off[t] = r2; r2 += s_cnt[tbase + t]; }
+            off[FUSED_ROWS] = r2;
+            __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-        const int64_t prefix = s_prefix;
-        const int64_t total = s_off[FUSED_ROWS];
-        if (sub == 0 && i < g.m) g.Cp[i] = prefix + s_off[team];
-        if (tile == ntiles - 1 && threadIdx.x == 0) g.Cp[g.m] = prefix + total;
+        int64_t prefix = 0;
+        if (tile > 0)
+        {
+            // window positions u*64 + lane = distance - 1; all loads are
+            // independent (relaxed: the counts travel inside the status words)
+            int64_t end = tile - 1;
+            int spins = 0;
+            while (true)
+            {
+                unsigned long long w[LB_WIN];
+                #pragma unroll
+                for (int u = 0; u < LB_WIN; u++)
+                {
+                    const int64_t idx = end - (int64_t) u * 64 - lane;
+                    w[u] = (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : LB_PRE;
+                }
+                int firstp = LB_WIN * 64;
+                bool zero_before = false;
+                #pragma unroll
+                for (int u = 0; u < LB_WIN; u++)
+                {
+                    const unsigned long long flag = w[u] & ~LB_MASK;
+                    const unsigned long long bp = __ballot(flag == LB_PRE);
+                    const unsigned long long b0 = __ballot(flag == 0);
+                    if (firstp == LB_WIN * 64)
+                    {
+                        const int fp = bp ? (__ffsll((long long) bp) - 1) : 64;
+                        const unsigned long long upto =
+                            (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
+                        if (b0 & upto) zero_before = true;
+                        if (fp < 64) firstp = u * 64 + fp;
+                    }
+                }
+                if (zero_before)
+                {
+                    // a predecessor has not published yet
+                    if (++spins > LB_SPIN_LIMIT)
+                    {
+                        if (lane == 0) atomicExch(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                int64_t v = 0;
+                #pragma unroll
+                for (int u = 0; u < LB_WIN; u++)
+                    if (u * 64 + lane <= firstp) v += (int64_t) (w[u] & LB_MASK);
+                #pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                prefix += v;
+                if (firstp < LB_WIN * 64) break;
+                end -= LB_WIN * 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        wave_sync();
+        const int64_t total = off[FUSED_ROWS];
+        if (sub == 0 && i < g.m) g.Cp[i] = prefix + off[tw];
+        if (tile == ntiles - 1 && lane == 0) g.Cp[g.m] = prefix + total;
         // contiguous copy of the tile's small-row output
-        for (int64_t e = threadIdx.x; e < total; e += SX_NT)
+        for (int64_t e = lane; e < total; e += 64)
         {
             int lo = 0, hi = FUSED_ROWS - 1;
             while (lo < hi)
             {
                 const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
+                if (off[mid] <= e) lo = mid; else hi = mid - 1;
             }
-            if (s_big[lo]) continue;
-            const int pos = (int) (e - s_off[lo]);
-            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[lo])[pos];
-            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[lo][pos];
+            if (s_big[tbase + lo]) continue;
+            const int pos = (int) (e - off[lo]);
+            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[tbase + lo])[pos];
+            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[tbase + lo][pos];
         }
-        __syncthreads();
+        // the next tile of this wavefront reuses its LDS rows
+        wave_sync();
     }
 }
 
@@ -1124,14 +1126,15 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     g.rows = rows.as<uint32_t>(); g.cnt = cnt.as<unsigned long long>();
     g.gtab_off = goff.as<int64_t>(); g.gtab_log2 = glog.as<int32_t>();
     g.gkeys = nullptr; g.gvals = nullptr; g.glist = nullptr;
-    g.ws_max = (prm && prm->wave_small_max > 0) ? prm->wave_small_max : 64;
-    g.wl_max = (prm && prm->wave_large_max > 0) ? prm->wave_large_max : 512;
-    g.blk_max = (prm && prm->block_max > 0) ? prm->block_max : 2048;
+    const gm_saxpy3_params tu = saxpy3_tuned();
+    g.ws_max = pick_param(prm ? prm->wave_small_max : 0, tu.wave_small_max, 64);
+    g.wl_max = pick_param(prm ? prm->wave_large_max : 0, tu.wave_large_max, 512);
+    g.blk_max = pick_param(prm ? prm->block_max : 0, tu.block_max, 2048);
     if (g.ws_max > FMAX) g.ws_max = FMAX;
     if (g.wl_max > 512) g.wl_max = 512;
     if (g.blk_max > 2048) g.blk_max = 2048;
     const int64_t fs = g.ws_max;        // flops bound of a "small" row
-    const int mode = prm ? (int) prm->mode : 0;
+    const int mode = (prm && prm->mode) ? (int) prm->mode : (int) tu.mode;
     // the fused kernel holds C's small rows at their flop bound until the
     // scan; it is used while that bound stays within a third of HBM
     const double budget = (double) device_hbm_bytes() / 3.0;

@@ -5,23 +5,51 @@
 //   GB_mxm       Source/mxm/GB_mxm.c:24-222   (types, dimensions, transplant)
 //   GB_AxB_meta  Source/mxm/GB_AxB_meta.c:41-875 (orientation, method choice)
 // With every matrix stored by row, the reference's CSR folding and swap rule
-// (GB_AxB_meta.c:214-447) end in two kernels without explicit transposes:
+// (GB_AxB_meta.c:214-447) end in two kernels:
 //   C<M> = A*B'  with M present and not complemented  -> dot3
 //                (GB_AxB_dot3_control, Source/mxm/GB_mxm.h:233-241)
 //   C = A*B, C<M> = A*B, C<!M> = A*B                   -> saxpy3
 //                (GB_AxB_saxpy_sparsity.c:56-83, default policy)
-// Other shapes (A' operands, an unmasked or complemented A*B', accum, or a
-// masked update of a non-empty C without GrB_REPLACE) return
-// GrB_NOT_IMPLEMENTED: there is no CPU path behind this library.
+// An A' operand, and a B' operand that does not go to dot3, are transposed
+// on the device first (gm_transpose; the reference's explicit transposes of
+// GB_AxB_meta.c:683-694,718-723).  The product T then reaches C by
+// transplant, or through gm_accum_mask (GB_accum_mask) when there is an
+// accumulator or a mask over a non-empty C without GrB_REPLACE.  A result
+// typecast returns GrB_NOT_IMPLEMENTED: there is no CPU path behind this
+// library.
 
 #include "grb_internal.h"
 
 GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran, int axb)
 {
     (void) axb;
-    if (a_tran) return GB_METHOD_NONE;
-    if (b_tran) return (has_mask && !mask_comp) ? GB_METHOD_DOT3 : GB_METHOD_NONE;
+    (void) a_tran;      // an A' operand is transposed first; the choice is the same
+    // C<M> = A*B' -> dot3; every other shape (after transposing B' when
+    // needed) -> saxpy3
+    if (b_tran && has_mask && !mask_comp) return GB_METHOD_DOT3;
     return GB_METHOD_SAXPY3;
+}
+
+// the gm_csr view of a device result (no copy)
+static gm_csr GB_view_of(const gm_csr_out *R)
+{
+    gm_csr v;
+    v.nrows = R->nrows;
+    v.ncols = R->ncols;
+    v.p = R->p;
+    v.j = R->j;
+    v.x = R->x;
+    v.iso = R->iso;
+    v.type = R->type;
+    return v;
+}
+
+static void GB_free_out(gm_csr_out *R)
+{
+    gm_free(R->p);
+    gm_free(R->j);
+    gm_free(R->x);
+    memset(R, 0, sizeof(*R));
 }
 
 static const char *GB_sr_name(const gm_semiring *s, char *buf, size_t len)
@@ -89,18 +117,23 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
     if (valued && (A->type->code != sr.ztype || B->type->code != sr.ztype))
         return GB_error(C, GrB_NOT_IMPLEMENTED,
                         "GrB_mxm: A and B must have the semiring's type %s", C->type->name);
+    // accumulator: a built-in operator on C's type (no typecast)
+    int32_t accum_op = 0;
     if (accum)
-        return GB_error(C, GrB_NOT_IMPLEMENTED, "GrB_mxm: accum is not supported");
-    if (Mask && !C_replace && C->nvals > 0)
-        return GB_error(C, GrB_NOT_IMPLEMENTED,
-                        "GrB_mxm: masked update of a non-empty C needs GrB_REPLACE");
+    {
+        if (accum->ztype != C->type || accum->xtype != C->type || accum->ytype != C->type ||
+            accum->opcode < GB_OP_PLUS || accum->opcode > GB_OP_LAND)
+            return GB_error(C, GrB_NOT_IMPLEMENTED,
+                            "GrB_mxm: accum %s on C of type %s is not supported", accum->name,
+                            C->type->name);
+        accum_op = (int32_t) accum->opcode;
+    }
+    // T reaches C through GB_accum_mask when C's old entries matter
+    const bool post = C->nvals > 0 && (accum != NULL || (Mask != NULL && !C_replace));
 
     const GB_method method = GB_mxm_select(Mask != NULL, mask_comp, a_tran, b_tran, axb);
-    if (method == GB_METHOD_NONE)
-        return GB_error(C, GrB_NOT_IMPLEMENTED,
-                        "GrB_mxm: operand orientation needs an explicit transpose (not supported)");
 
-    gm_csr va, vb, vm;
+    gm_csr va, vb, vm, vc;
     if ((info = GB_matrix_view(&va, A)) != GrB_SUCCESS) return GB_error(C, info, "%s", A->err);
     if ((info = GB_matrix_view(&vb, B)) != GrB_SUCCESS) return GB_error(C, info, "%s", B->err);
     if (Mask && (info = GB_matrix_view(&vm, Mask)) != GrB_SUCCESS)
@@ -109,10 +142,30 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
     // as all true (structural)
     const bool mstruct = mask_struct || (Mask && Mask->x == NULL);
 
-    gm_csr_out R;
+    gm_csr_out R, AT, BT;
     memset(&R, 0, sizeof(R));
-    int rc;
+    memset(&AT, 0, sizeof(AT));
+    memset(&BT, 0, sizeof(BT));
+    int rc = GM_SUCCESS;
     char nm[64];
+    if (a_tran)
+    {
+        if (GB_burble) printf("burble: GrB_mxm: transpose A (%lld entries)\n", (long long) A->nvals);
+        rc = gm_transpose(&AT, &va);
+        if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+        va = GB_view_of(&AT);
+    }
+    if (b_tran && method == GB_METHOD_SAXPY3)
+    {
+        if (GB_burble) printf("burble: GrB_mxm: transpose B (%lld entries)\n", (long long) B->nvals);
+        rc = gm_transpose(&BT, &vb);
+        if (rc != GM_SUCCESS)
+        {
+            GB_free_out(&AT);
+            return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+        }
+        vb = GB_view_of(&BT);
+    }
     if (method == GB_METHOD_DOT3)
     {
         if (GB_burble)
@@ -126,10 +179,34 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         if (GB_burble)
             printf("burble: GrB_mxm C%s=A*B %s: saxpy3 (hash), A nnz %lld B nnz %lld\n",
                    Mask ? (mask_comp ? "<!M>" : "<M>") : "", GB_sr_name(&sr, nm, sizeof(nm)),
-                   (long long) A->nvals, (long long) B->nvals);
+                   (long long) (a_tran ? AT.nvals : A->nvals),
+                   (long long) (b_tran ? BT.nvals : B->nvals));
         rc = gm_saxpy3(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, NULL);
     }
+    GB_free_out(&AT);
+    GB_free_out(&BT);
     if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+    if (post)
+    {
+        // C<M, replace> = accum (C, T) on the device (GB_mxm.c:201-214 -> GB_accum_mask)
+        if (GB_burble)
+            printf("burble: GrB_mxm: accum/mask into C (%lld entries, accum op %d)\n",
+                   (long long) C->nvals, (int) accum_op);
+        gm_csr_out O;
+        memset(&O, 0, sizeof(O));
+        R.type = C->type->code;
+        if (R.iso && R.x == NULL) R.iso = 0;
+        const gm_csr vt = GB_view_of(&R);
+        if ((info = GB_matrix_view(&vc, C)) != GrB_SUCCESS)
+        {
+            GB_free_out(&R);
+            return GB_error(C, info, "%s", C->err);
+        }
+        rc = gm_accum_mask(&O, &vc, Mask ? &vm : NULL, mask_comp, mstruct, C_replace, &vt, accum_op);
+        GB_free_out(&R);
+        if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+        R = O;
+    }
     // transplant T into C (GB_mxm.c:168-200)
     GB_matrix_set_csr(C, &R);
     C->err[0] = '\0';

@@ -165,6 +165,18 @@ int gm_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int gm_memset(void *dst, int value, size_t bytes);
 int gm_synchronize(void);
 
+// Process-wide tuned values for the fields of gm_dot3_params /
+// gm_saxpy3_params left 0 (and for calls that pass NULL, as GrB_mxm does);
+// NULL restores the built-in values.  gm_load_tuning reads "key value" lines
+// (dot3.thread_max, dot3.team16_max, dot3.wave_max, dot3.bitmap_min_len,
+// dot3.order, saxpy3.wave_small_max, saxpy3.wave_large_max,
+// saxpy3.block_max, saxpy3.mode; '#' starts a comment line) as written by
+// the autotuner (graphblas_amd/tune.py); gm_init loads the file named by the
+// environment variable GM_TUNING_FILE.  No counterpart in the reference.
+int gm_dot3_set_defaults(const gm_dot3_params *p);
+int gm_saxpy3_set_defaults(const gm_saxpy3_params *p);
+int gm_load_tuning(const char *path);
+
 // per-kernel timing with HIP events on the library stream
 void gm_prof_enable(int on);
 void gm_prof_reset(void);
@@ -192,6 +204,24 @@ bool gm_saxpy3_branch(const gm_csr *M, bool mask_comp, bool mask_struct,
 int gm_saxpy3(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct,
               const gm_csr *A, const gm_csr *B, const gm_semiring *sr,
               const gm_saxpy3_params *params);
+
+//------------------------------------------------------------------------------
+// steps around the products: transpose, accumulate and mask
+//------------------------------------------------------------------------------
+
+// T = A' (CSR of the transpose, rows sorted; an iso A gives an iso T).
+// Replaces GB_transpose (Source/transpose/GB_transpose.c) where the
+// orientation rules of GB_AxB_meta.c:683-694,718-723 need an explicit one.
+int gm_transpose(gm_csr_out *T, const gm_csr *A);
+
+// O = C<M, replace> = accum (C, T), the reference's GB_accum_mask
+// (Source/mask/GB_accum_mask.c) for sparse operands of one type (bool,
+// int32, int64, fp64).  C may be NULL (empty); M may be NULL (no mask).
+// accum_op: 0 none (C<M> = T), else the reference's opcode order
+// 1 PLUS, 2 MINUS, 3 TIMES, 4 MIN, 5 MAX, 6 FIRST, 7 SECOND, 8 ANY, 9 PAIR,
+// 10 LOR, 11 LAND.  Outside the mask C's entries stay unless replace.
+int gm_accum_mask(gm_csr_out *O, const gm_csr *C, const gm_csr *M, bool mask_comp,
+                  bool mask_struct, bool replace, const gm_csr *T, int32_t accum_op);
 
 //------------------------------------------------------------------------------
 // batches: many independent requests in one launch sequence (no counterpart

@@ -136,3 +136,103 @@ def set_threads(n: int):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+# ---------------------------------------------------------------------------
+# steps around the product (small inputs: plain Python over the entries)
+# ---------------------------------------------------------------------------
+
+def transpose(A):
+    """CSR of A' with sorted rows: the column-major order of A's entries
+    (the reference's GB_transpose, Source/transpose/GB_transpose.c, builds
+    the same matrix by bucket sort)."""
+    from graphblas_amd.workloads import Csr
+    rows = np.repeat(np.arange(A.nrows, dtype=np.int64), np.diff(A.p))
+    order = np.lexsort((rows, A.j))          # by column, then by row
+    Tj = rows[order]
+    counts = np.bincount(A.j, minlength=A.ncols) if A.nnz else np.zeros(A.ncols, dtype=np.int64)
+    Tp = np.zeros(A.ncols + 1, dtype=np.int64)
+    Tp[1:] = np.cumsum(counts)
+    Tx = None
+    if A.x is not None:
+        Tx = A.x[order] if not getattr(A, "iso", False) else A.x
+    return Csr(A.ncols, A.nrows, Tp, Tj, Tx)
+
+
+def _accum_op(op: int, c, t, dtype):
+    """z = accum(c, t) for the reference's built-in opcodes (PLUS=1 .. LAND=11;
+    the semantics of Source/builtin/include/GB_binop*.h for one type).
+    Integer arithmetic wraps; bool PLUS/MAX are OR, TIMES/MIN are AND, MINUS is XOR."""
+    if dtype == np.uint8:
+        a, b = bool(c), bool(t)
+        r = {1: a or b, 2: a != b, 3: a and b, 4: a and b, 5: a or b, 6: a, 7: b, 8: b, 9: True,
+             10: a or b, 11: a and b}[op]
+        return np.uint8(r)
+    if dtype == np.float64:
+        a, b = float(c), float(t)
+
+        def fmin(x, y):
+            return y if x != x else (x if y != y else min(x, y))
+
+        def fmax(x, y):
+            return y if x != x else (x if y != y else max(x, y))
+        r = {1: lambda: a + b, 2: lambda: a - b, 3: lambda: a * b, 4: lambda: fmin(a, b),
+             5: lambda: fmax(a, b), 6: lambda: a, 7: lambda: b, 8: lambda: b, 9: lambda: 1.0,
+             10: lambda: float(a != 0 or b != 0), 11: lambda: float(a != 0 and b != 0)}[op]()
+        return np.float64(r)
+    bits = np.dtype(dtype).itemsize * 8
+    a, b = int(c), int(t)
+    r = {1: a + b, 2: a - b, 3: a * b, 4: min(a, b), 5: max(a, b), 6: a, 7: b, 8: b, 9: 1,
+         10: int(a != 0 or b != 0), 11: int(a != 0 and b != 0)}[op]
+    r &= (1 << bits) - 1
+    if r >= 1 << (bits - 1):
+        r -= 1 << bits
+    return dtype(r)
+
+
+def accum_mask(C, M, T, mask_comp=False, mask_struct=True, replace=False, op=0, dtype=np.int64):
+    """C<M, replace> = accum(C, T), the reference's GB_accum_mask
+    (Source/mask/GB_accum_mask.c) with the GraphBLAS spec's rules: Z = T
+    (no accum) or the union of C and T with accum on the intersection; inside
+    the mask C takes Z, outside C keeps its entries unless replace.  C, M may
+    be None."""
+    from graphblas_amd.workloads import Csr
+
+    def entries(A, iso=False):
+        d = {}
+        if A is None:
+            return d
+        for i in range(A.nrows):
+            for p in range(A.p[i], A.p[i + 1]):
+                d[(i, int(A.j[p]))] = None if A.x is None else A.x[0 if iso else p]
+        return d
+    ce = entries(C, getattr(C, "iso", False) if C is not None else False)
+    te = entries(T, getattr(T, "iso", False))
+    me = entries(M, getattr(M, "iso", False)) if M is not None else None
+    out = {}
+    for key in set(ce) | set(te):
+        if me is None:
+            allow = True
+        else:
+            mij = key in me and (mask_struct or me[key] is None or me[key] != 0)
+            allow = (not mij) if mask_comp else mij
+        cin, tin = key in ce, key in te
+        if allow:
+            if op:
+                if cin and tin:
+                    out[key] = _accum_op(op, ce[key], te[key], dtype)
+                elif cin or tin:
+                    out[key] = ce[key] if cin else te[key]
+            elif tin:
+                out[key] = te[key]
+        elif cin and not replace:
+            out[key] = ce[key]
+    m = T.nrows
+    keys = sorted(out)
+    p = np.zeros(m + 1, dtype=np.int64)
+    for (i, _) in keys:
+        p[i + 1] += 1
+    p = np.cumsum(p)
+    j = np.array([k[1] for k in keys], dtype=np.int64)
+    x = np.array([out[k] for k in keys], dtype=dtype)
+    return Csr(m, T.ncols, p, j, x)

@@ -67,14 +67,15 @@ def test_semiring_decode():
 
 def test_method_selection():
     sel = capi.lib().GB_mxm_select
-    # (mask, comp, a_tran, b_tran) -> 1 dot3, 2 saxpy3, 0 not handled
+    # (mask, comp, a_tran, b_tran) -> 1 dot3, 2 saxpy3 (B' transposed first)
     assert sel(True, False, False, True, 0) == 1       # C<M>=A*B'  (tricount, ST1)
-    assert sel(True, True, False, True, 0) == 0        # C<!M>=A*B' needs dot2
-    assert sel(False, False, False, True, 0) == 0      # C=A*B' needs a transpose
+    assert sel(True, True, False, True, 0) == 2        # C<!M>=A*B': transpose B, saxpy3
+    assert sel(False, False, False, True, 0) == 2      # C=A*B': transpose B, saxpy3
     assert sel(False, False, False, False, 0) == 2     # C=A*B
     assert sel(True, False, False, False, 0) == 2      # C<M>=A*B
     assert sel(True, True, False, False, 0) == 2       # C<!M>=A*B (BFS, RSC)
-    assert sel(True, False, True, False, 0) == 0       # A' operand
+    assert sel(True, False, True, False, 0) == 2       # C<M>=A'*B: transpose A, saxpy3
+    assert sel(True, False, True, True, 0) == 1        # C<M>=A'*B': transpose A, dot3
 
 
 def test_branch_predicates_host_only():
@@ -124,13 +125,14 @@ def test_grb_mxm_argument_errors_without_gpu():
     # result type must be C's type
     Cf = _mat("fp64", 4, 6)
     assert capi.mxm(Cf, M, sr, A, B, "ST1") == capi.GrB_NOT_IMPLEMENTED
-    # accum is outside this library
-    acc = capi.handle("GrB_PLUS_INT64")
+    # an accumulator of another type would need a typecast
+    acc = capi.handle("GrB_PLUS_FP64")
     assert L.GrB_mxm(C.h, M.h, acc, capi.handle(sr), A.h, B.h,
                      capi.desc("ST1")) == capi.GrB_NOT_IMPLEMENTED
-    # A' operand
+    # an A' operand reaches the device layer (a transpose there), which has
+    # no device in this container
     At = _mat("int64", 5, 4)
-    assert capi.mxm(C, M, sr, At, B, "ST0T1") == capi.GrB_NOT_IMPLEMENTED
+    assert capi.mxm(C, M, sr, At, B, "ST0T1") == capi.GrB_PANIC
     # mask dimension
     M2 = _mat("int64", 4, 7)
     assert capi.mxm(C, M2, sr, A, B, "ST1") == capi.GrB_DIMENSION_MISMATCH
@@ -199,3 +201,20 @@ def test_no_gpu_in_container_reports_cleanly():
     C = _mat("int64", 4, 4)
     rc = capi.mxm(C, None, "GxB_PLUS_PAIR_INT64", A, A)
     assert rc == capi.GrB_PANIC
+
+
+def test_tuning_file_parse(tmp_path):
+    """gm_load_tuning accepts the autotuner's key/value lines and rejects
+    unknown keys and missing files (host only, no GPU needed)."""
+    good = tmp_path / "t.txt"
+    good.write_text("# tuned\ndot3.thread_max 16\ndot3.order 2\nsaxpy3.block_max 1024\n")
+    try:
+        capi.load_tuning(str(good))
+        bad = tmp_path / "b.txt"
+        bad.write_text("dot3.nonsense 3\n")
+        with pytest.raises(capi.GrBError):
+            capi.load_tuning(str(bad))
+        with pytest.raises(capi.GrBError):
+            capi.load_tuning(str(tmp_path / "missing.txt"))
+    finally:
+        capi.reset_tuning()

@@ -20,6 +20,7 @@ import math
 import os
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import pyoracle as O
@@ -295,3 +296,77 @@ def test_planted_mistake_is_caught():
     ref = brute_dot3("plus_times_fp64", M, A, B)
     with pytest.raises(AssertionError):
         check_equal(wrong, ref, fp=True)
+
+
+# ---------------------------------------------------------------------------
+# transpose and accum/mask oracles
+# ---------------------------------------------------------------------------
+
+def _tiny(rows, ncols, dtype=np.int64):
+    """CSR from a list of {col: value} dicts."""
+    p = [0]
+    j, x = [], []
+    for r in rows:
+        for c in sorted(r):
+            j.append(c)
+            x.append(r[c])
+        p.append(len(j))
+    return W.Csr(len(rows), ncols, np.array(p, dtype=np.int64), np.array(j, dtype=np.int64),
+                 np.array(x, dtype=dtype))
+
+
+def _dense(A, fill=None):
+    out = {}
+    for i in range(A.nrows):
+        for q in range(A.p[i], A.p[i + 1]):
+            out[(i, int(A.j[q]))] = A.x[q].item() if A.x is not None else True
+    return out
+
+
+def test_transpose_oracle_matches_scipy_and_is_an_involution():
+    rng = np.random.default_rng(5)
+    for (m, n, d) in [(1, 1, 1.0), (7, 13, 0.3), (40, 25, 0.1), (30, 30, 0.0)]:
+        sm = sp.random(m, n, density=d, format="csr", random_state=rng)
+        sm.sort_indices()
+        A = W.Csr(m, n, sm.indptr.astype(np.int64), sm.indices.astype(np.int64), sm.data.copy())
+        T = O.transpose(A)
+        ref = sm.T.tocsr()
+        ref.sort_indices()
+        assert np.array_equal(T.p, ref.indptr) and np.array_equal(T.j, ref.indices)
+        assert np.array_equal(T.x, ref.data)
+        TT = O.transpose(T)
+        assert np.array_equal(TT.p, A.p) and np.array_equal(TT.j, A.j) and np.array_equal(TT.x, A.x)
+
+
+def test_accum_mask_oracle_hand_cases():
+    C = _tiny([{0: 1, 2: 3}, {1: 5}], 3)
+    T = _tiny([{0: 10, 1: 20}, {2: 30}], 3)
+    M = _tiny([{0: 1, 1: 1}, {0: 1, 2: 1}], 3)
+    # C<M> += T
+    got = _dense(O.accum_mask(C, M, T, op=1))
+    assert got == {(0, 0): 11, (0, 1): 20, (0, 2): 3, (1, 1): 5, (1, 2): 30}
+    # C<M, replace> = T
+    got = _dense(O.accum_mask(C, M, T, replace=True))
+    assert got == {(0, 0): 10, (0, 1): 20, (1, 2): 30}
+    # C<!M> min= T: only (0,2) and (1,1) are writable, T has nothing there
+    got = _dense(O.accum_mask(C, M, T, mask_comp=True, op=4))
+    assert got == {(0, 0): 1, (0, 2): 3, (1, 1): 5}
+    # no mask, SECOND: the union with T winning on the intersection
+    got = _dense(O.accum_mask(C, None, T, op=7))
+    assert got == {(0, 0): 10, (0, 1): 20, (0, 2): 3, (1, 1): 5, (1, 2): 30}
+    # valued mask: a zero entry does not allow
+    Mv = _tiny([{0: 0, 1: 1}, {0: 1, 2: 1}], 3)
+    got = _dense(O.accum_mask(C, Mv, T, mask_struct=False, op=1))
+    assert got == {(0, 0): 1, (0, 1): 20, (0, 2): 3, (1, 1): 5, (1, 2): 30}
+    # int32 wrap-around and fp64 / bool semantics
+    Ci = _tiny([{0: 2 ** 31 - 1}], 1, np.int32)
+    Ti = _tiny([{0: 1}], 1, np.int32)
+    assert _dense(O.accum_mask(Ci, None, Ti, op=1, dtype=np.int32)) == {(0, 0): -2 ** 31}
+    Cf = _tiny([{0: 1.5}], 1, np.float64)
+    Tf = _tiny([{0: 2.0}], 1, np.float64)
+    assert _dense(O.accum_mask(Cf, None, Tf, op=3, dtype=np.float64)) == {(0, 0): 3.0}
+    assert _dense(O.accum_mask(Cf, None, Tf, op=2, dtype=np.float64)) == {(0, 0): -0.5}
+    Cb = _tiny([{0: 1, 1: 0}], 2, np.uint8)
+    Tb = _tiny([{0: 1, 1: 1}], 2, np.uint8)
+    assert _dense(O.accum_mask(Cb, None, Tb, op=2, dtype=np.uint8)) == {(0, 0): 0, (0, 1): 1}
+    assert _dense(O.accum_mask(Cb, None, Tb, op=3, dtype=np.uint8)) == {(0, 0): 1, (0, 1): 0}
