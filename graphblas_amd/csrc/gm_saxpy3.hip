@@ -174,42 +174,6 @@ __device__ __forceinline__ T team_sum(T v)
 // max row lengths of A and B (decides whether every row is "small")
 //------------------------------------------------------------------------------
 
-// the same two maxima, with B's taken only over the rows that A references
-// (cheaper than a scan of B when A has far fewer rows than B, e.g. a few
-// frontier rows against a large graph); a lane per row, lanes stride its entries
-__global__ __launch_bounds__(SX_NT) void k_sx_maxlen_ref(const int64_t *__restrict__ Ap,
-                                                         const int64_t *__restrict__ Aj, int64_t m,
-                                                         const int64_t *__restrict__ Bp,
-                                                         unsigned long long *out)
-{
-    unsigned long long ma = 0, mb = 0;
-    for (int64_t i = (int64_t) blockIdx.x * SX_NT + threadIdx.x; i < m;
-         i += (int64_t) gridDim.x * SX_NT)
-    {
-        const int64_t p0 = Ap[i], p1 = Ap[i + 1];
-        const unsigned long long l = (unsigned long long) (p1 - p0);
-        ma = l > ma ? l : ma;
-        for (int64_t p = p0; p < p1; p++)
-        {
-            const int64_t k = Aj[p];
-            const unsigned long long lb = (unsigned long long) (Bp[k + 1] - Bp[k]);
-            mb = lb > mb ? lb : mb;
-        }
-    }
-    #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        unsigned long long ua = __shfl_xor(ma, off, 64), ub = __shfl_xor(mb, off, 64);
-        ma = ua > ma ? ua : ma;
-        mb = ub > mb ? ub : mb;
-    }
-    if ((threadIdx.x & 63) == 0)
-    {
-        atomicMax(&out[0], ma);
-        atomicMax(&out[1], mb);
-    }
-}
-
 __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__ Ap, int64_t m,
                                                      const int64_t *__restrict__ Bp, int64_t kb,
                                                      unsigned long long *out)
@@ -1156,33 +1120,34 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         return GM_SUCCESS;
     }
 
-    // 1. max row lengths of A and B, and nnz(A)
+    // 1. max row lengths of A and B, and nnz(A), for the all-small shortcut.
+    // When A has far fewer rows than B (a few frontier rows against a large
+    // graph) the flop pass below is as cheap as this one, so it is skipped
+    // together with its synchronisation.
+    int64_t maxA = 0, maxB = 0, nnzA = 0;
+    const bool probe_small = m * 8 >= B->nrows;
     k_sx_fill_u64<<<1, 64, 0, st>>>(g.cnt, 16);
     GM_HIP(hipGetLastError(), "k_sx_fill_u64");
+    if (probe_small)
     {
-        ProfScope ps("sx_maxlen");
-        if (m * 8 < B->nrows)
         {
-            k_sx_maxlen_ref<<<sx_grid(m, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
-                A->p, A->j, m, B->p, g.cnt + 8);
-            GM_HIP(hipGetLastError(), "k_sx_maxlen_ref");
-        }
-        else
-        {
+            ProfScope ps("sx_maxlen");
             const int64_t lim = m > B->nrows ? m : B->nrows;
             k_sx_maxlen<<<sx_grid(lim, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
                 A->p, m, B->p, B->nrows, g.cnt + 8);
             GM_HIP(hipGetLastError(), "k_sx_maxlen");
         }
+        GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+               "maxlen d2h");
+        GM_HIP(hipMemcpyAsync(hp + 10, A->p + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
+        GM_HIP(hipStreamSynchronize(st), "maxlen sync");
+        maxA = (int64_t) hp[8];
+        maxB = (int64_t) hp[9];
+        nnzA = (int64_t) hp[10];
     }
-    GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
-           "maxlen d2h");
-    GM_HIP(hipMemcpyAsync(hp + 10, A->p + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
-    GM_HIP(hipStreamSynchronize(st), "maxlen sync");
-    const int64_t maxA = (int64_t) hp[8], maxB = (int64_t) hp[9], nnzA = (int64_t) hp[10];
 
     bool fused = (mode != 2);
-    bool all_small = fused && (double) maxA * (double) maxB <= (double) fs &&
+    bool all_small = fused && probe_small && (double) maxA * (double) maxB <= (double) fs &&
                      (double) nnzA * (double) maxB * (double) entry_bytes <= budget;
     int64_t cap_small = all_small ? nnzA * maxB : 0;
 
