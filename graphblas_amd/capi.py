@@ -233,6 +233,23 @@ def lib():
                             ctypes.c_int),
         "gm_batch_split": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsrOut), P, P, I64],
                            ctypes.c_int),
+        "GrB_Vector_new": ([PP, P, U64], ctypes.c_int),
+        "GrB_Vector_free": ([PP], ctypes.c_int),
+        "GrB_Vector_nvals": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_size": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_clear": ([P], ctypes.c_int),
+        "GrB_Vector_build_INT64": ([P, P, P, U64, P], ctypes.c_int),
+        "GrB_Vector_build_INT32": ([P, P, P, U64, P], ctypes.c_int),
+        "GrB_Vector_build_FP64": ([P, P, P, U64, P], ctypes.c_int),
+        "GrB_Vector_build_BOOL": ([P, P, P, U64, P], ctypes.c_int),
+        "GrB_Vector_extractTuples_INT64": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_extractTuples_INT32": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_extractTuples_FP64": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_extractTuples_BOOL": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Vector_reduce_INT64": ([ctypes.POINTER(I64), P, P, P, P], ctypes.c_int),
+        "GrB_Vector_error": ([ctypes.POINTER(ctypes.c_char_p), P], ctypes.c_int),
+        "GrB_vxm": ([P, P, P, P, P, P, P], ctypes.c_int),
+        "GrB_mxv": ([P, P, P, P, P, P, P], ctypes.c_int),
         "gm_transpose": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr)], ctypes.c_int),
         "gm_accum_mask": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                            ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr),
@@ -389,6 +406,67 @@ class Matrix:
                 if q.value:
                     _libc.free(q)
             lib().GrB_Matrix_free(ctypes.byref(D))
+
+
+_VT = {"int64": "INT64", "int32": "INT32", "fp64": "FP64", "bool": "BOOL"}
+
+
+class Vector:
+    """A GrB_Vector (held in HBM as a 1 x n CSR row)."""
+
+    def __init__(self, type_name: str, n: int):
+        init()
+        self.type_name = type_name
+        self.n = n
+        h = ctypes.c_void_p()
+        _check(lib().GrB_Vector_new(ctypes.byref(h), handle(GRB_TYPE_NAMES[type_name]), n),
+               "GrB_Vector_new")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().GrB_Vector_free(ctypes.byref(self.h))
+        except Exception:
+            pass
+
+    @classmethod
+    def from_arrays(cls, idx, vals, type_name: str, n: int) -> "Vector":
+        v = cls(type_name, n)
+        I = np.ascontiguousarray(idx, dtype=np.uint64)
+        X = np.ascontiguousarray(vals, dtype=NP_TYPES[type_name])
+        f = getattr(lib(), "GrB_Vector_build_" + _VT[type_name])
+        _check(f(v.h, I.ctypes.data, X.ctypes.data, len(I), None), "GrB_Vector_build")
+        return v
+
+    @property
+    def nvals(self) -> int:
+        n = ctypes.c_uint64()
+        _check(lib().GrB_Vector_nvals(ctypes.byref(n), self.h), "GrB_Vector_nvals")
+        return n.value
+
+    def to_arrays(self):
+        nv = self.nvals
+        I = np.empty(max(nv, 1), dtype=np.uint64)
+        X = np.empty(max(nv, 1), dtype=NP_TYPES[self.type_name])
+        n = ctypes.c_uint64(nv)
+        f = getattr(lib(), "GrB_Vector_extractTuples_" + _VT[self.type_name])
+        _check(f(I.ctypes.data, X.ctypes.data, ctypes.byref(n), self.h), "GrB_Vector_extractTuples")
+        return I[:n.value].astype(np.int64), X[:n.value]
+
+
+def vxm(w: Vector, mask: Optional[Vector], semiring: str, u: Vector, A: Matrix,
+        descriptor: Optional[str] = None, accum: Optional[str] = None) -> int:
+    return lib().GrB_vxm(w.h, mask.h if mask is not None else None,
+                         handle(accum) if accum else None, handle(semiring), u.h, A.h,
+                         desc(descriptor))
+
+
+def mxv(w: Vector, mask: Optional[Vector], semiring: str, A: Matrix, u: Vector,
+        descriptor: Optional[str] = None, accum: Optional[str] = None) -> int:
+    return lib().GrB_mxv(w.h, mask.h if mask is not None else None,
+                         handle(accum) if accum else None, handle(semiring), A.h, u.h,
+                         desc(descriptor))
 
 
 def mxm(C: Matrix, M: Optional[Matrix], semiring: str, A: Matrix, B: Matrix,

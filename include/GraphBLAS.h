@@ -74,6 +74,7 @@ typedef struct GB_Monoid_opaque *GrB_Monoid;
 typedef struct GB_Semiring_opaque *GrB_Semiring;
 typedef struct GB_Descriptor_opaque *GrB_Descriptor;
 typedef struct GB_Matrix_opaque *GrB_Matrix;
+typedef struct GB_Vector_opaque *GrB_Vector;
 
 extern GrB_Type GrB_BOOL, GrB_INT8, GrB_UINT8, GrB_INT16, GrB_UINT16, GrB_INT32,
     GrB_UINT32, GrB_INT64, GrB_UINT64, GrB_FP32, GrB_FP64;
@@ -212,6 +213,33 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
                                bool *iso, bool *jumbled, const GrB_Descriptor desc);
 
 //------------------------------------------------------------------------------
+// sparse vectors (Include/GraphBLAS.h:3623-3670, 3745, 4197 of the reference)
+//------------------------------------------------------------------------------
+
+GrB_Info GrB_Vector_new(GrB_Vector *v, GrB_Type type, GrB_Index n);
+GrB_Info GrB_Vector_dup(GrB_Vector *w, const GrB_Vector u);
+GrB_Info GrB_Vector_clear(GrB_Vector v);
+GrB_Info GrB_Vector_size(GrB_Index *n, const GrB_Vector v);
+GrB_Info GrB_Vector_nvals(GrB_Index *nvals, const GrB_Vector v);
+GrB_Info GrB_Vector_free(GrB_Vector *v);
+GrB_Info GrB_Vector_build_BOOL(GrB_Vector w, const GrB_Index *I, const bool *X,
+                               GrB_Index nvals, const GrB_BinaryOp dup);
+GrB_Info GrB_Vector_build_INT32(GrB_Vector w, const GrB_Index *I, const int32_t *X,
+                                GrB_Index nvals, const GrB_BinaryOp dup);
+GrB_Info GrB_Vector_build_INT64(GrB_Vector w, const GrB_Index *I, const int64_t *X,
+                                GrB_Index nvals, const GrB_BinaryOp dup);
+GrB_Info GrB_Vector_build_FP64(GrB_Vector w, const GrB_Index *I, const double *X,
+                               GrB_Index nvals, const GrB_BinaryOp dup);
+GrB_Info GrB_Vector_extractTuples_BOOL(GrB_Index *I, bool *X, GrB_Index *nvals,
+                                       const GrB_Vector v);
+GrB_Info GrB_Vector_extractTuples_INT32(GrB_Index *I, int32_t *X, GrB_Index *nvals,
+                                        const GrB_Vector v);
+GrB_Info GrB_Vector_extractTuples_INT64(GrB_Index *I, int64_t *X, GrB_Index *nvals,
+                                        const GrB_Vector v);
+GrB_Info GrB_Vector_extractTuples_FP64(GrB_Index *I, double *X, GrB_Index *nvals,
+                                       const GrB_Vector v);
+
+//------------------------------------------------------------------------------
 // C<Mask> = accum (C, A*B): the hot path
 //------------------------------------------------------------------------------
 
@@ -226,6 +254,16 @@ GrB_Info GrB_mxm
     const GrB_Descriptor desc       // descriptor for C, Mask, A, and B
 );
 
+// w'<mask'> = accum (w', u'*A) and w<mask> = accum (w, A*u)
+// (Include/GraphBLAS.h:5760-5780 of the reference); they run on GrB_mxm's
+// kernels with the vectors as 1 x n matrices
+GrB_Info GrB_vxm(GrB_Vector w, const GrB_Vector mask, const GrB_BinaryOp accum,
+                 const GrB_Semiring semiring, const GrB_Vector u, const GrB_Matrix A,
+                 const GrB_Descriptor desc);
+GrB_Info GrB_mxv(GrB_Vector w, const GrB_Vector mask, const GrB_BinaryOp accum,
+                 const GrB_Semiring semiring, const GrB_Matrix A, const GrB_Vector u,
+                 const GrB_Descriptor desc);
+
 //------------------------------------------------------------------------------
 // reduce a matrix to a scalar: c = accum (c, reduce (A))
 //------------------------------------------------------------------------------
@@ -238,13 +276,27 @@ GrB_Info GrB_Matrix_reduce_FP64(double *c, const GrB_BinaryOp accum, const GrB_M
                                 const GrB_Matrix A, const GrB_Descriptor desc);
 GrB_Info GrB_Matrix_reduce_BOOL(bool *c, const GrB_BinaryOp accum, const GrB_Monoid monoid,
                                 const GrB_Matrix A, const GrB_Descriptor desc);
+GrB_Info GrB_Vector_reduce_INT64(int64_t *c, const GrB_BinaryOp accum, const GrB_Monoid monoid,
+                                 const GrB_Vector u, const GrB_Descriptor desc);
+GrB_Info GrB_Vector_reduce_INT32(int32_t *c, const GrB_BinaryOp accum, const GrB_Monoid monoid,
+                                 const GrB_Vector u, const GrB_Descriptor desc);
+GrB_Info GrB_Vector_reduce_FP64(double *c, const GrB_BinaryOp accum, const GrB_Monoid monoid,
+                                const GrB_Vector u, const GrB_Descriptor desc);
+GrB_Info GrB_Vector_reduce_BOOL(bool *c, const GrB_BinaryOp accum, const GrB_Monoid monoid,
+                                const GrB_Vector u, const GrB_Descriptor desc);
 
 //------------------------------------------------------------------------------
 // errors and global options
 //------------------------------------------------------------------------------
 
 GrB_Info GrB_Matrix_error(const char **error, const GrB_Matrix A);
+GrB_Info GrB_Vector_error(const char **error, const GrB_Vector v);
+#if defined(__cplusplus)
 #define GrB_error(error, object) GrB_Matrix_error(error, object)
+#else
+#define GrB_error(error, object) \
+    _Generic((object), GrB_Vector: GrB_Vector_error, default: GrB_Matrix_error)(error, object)
+#endif
 
 #define GxB_BURBLE 7019   // GxB_Option_Field value in the reference
 GrB_Info GxB_Global_Option_set_INT32(int field, int32_t value);

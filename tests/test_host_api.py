@@ -218,3 +218,19 @@ def test_tuning_file_parse(tmp_path):
             capi.load_tuning(str(tmp_path / "missing.txt"))
     finally:
         capi.reset_tuning()
+
+
+def test_vector_objects_without_gpu():
+    """Vector handles, sizes and argument errors need no device."""
+    capi.init()
+    L = capi.lib()
+    v = capi.Vector("int64", 10)
+    n = ctypes.c_uint64()
+    assert L.GrB_Vector_size(ctypes.byref(n), v.h) == 0 and n.value == 10
+    assert v.nvals == 0
+    A = _mat("int64", 10, 10)
+    assert L.GrB_vxm(None, None, None, capi.handle("GxB_PLUS_PAIR_INT64"), v.h, A.h, None) == \
+        capi.GrB_NULL_POINTER
+    w = capi.Vector("int64", 7)
+    # u' (1 x 10) * A (10 x 10) cannot go into a vector of size 7
+    assert capi.vxm(w, None, "GxB_PLUS_PAIR_INT64", v, A) == capi.GrB_DIMENSION_MISMATCH
