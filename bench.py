@@ -56,12 +56,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="saxpy3-er20",
                     choices=["saxpy3-er20", "dot3-rmat14", "dot3-rmat22", "served-rmat20",
-                             "batched-4096"])
+                             "batched-4096", "saxpy3-tricount-rmat22"])
     ap.add_argument("--requests", type=int, default=4096,
                     help="served workload: number of requests")
-    ap.add_argument("--rate", type=float, default=0.0,
-                    help="served workload: Poisson arrival rate in requests/s (0: half the "
-                         "measured saturation throughput of the served path)")
+    ap.add_argument("--rate", type=float, default=10000.0,
+                    help="served workload: Poisson arrival rate in requests/s")
+    ap.add_argument("--tuning", default=os.path.join("graphblas_amd", "tuning", "mi355x_batched.txt"),
+                    help="batched workload: autotuned bin cutoffs (gm_load_tuning); '' for none")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="budget for the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -187,7 +188,7 @@ def bench_served(args, capi, W):
     t_srv, r_srv, st_srv = run_server(capi, W, dG.view, reqs, 0, 2, 256, budget)
     t_pl, r_pl, st_pl = run_server(capi, W, dG.view, reqs, 0, 1, 1, budget)
     sat = len(reqs) / t_srv
-    rate = args.rate if args.rate > 0 else 0.5 * sat
+    rate = args.rate
     # open loop: Poisson arrivals at the fixed rate
     t_po, r_po, st_po = run_server(capi, W, dG.view, reqs, rate, 2, 256, budget)
     t_pp, r_pp, st_pp = run_server(capi, W, dG.view, reqs, rate, 1, 1, budget)
@@ -248,6 +249,10 @@ def bench_batched(args, capi, W):
     the plain path (one gm_dot3 per request)."""
     import ctypes
     L = capi.lib()
+    tuning = None
+    if args.tuning:
+        tuning = os.path.join(ROOT, args.tuning) if not os.path.isabs(args.tuning) else args.tuning
+        capi.load_tuning(tuning)
     nreq = args.requests
     specs = W.batch_specs(nreq, seed=1)
     t0 = time.perf_counter()
@@ -349,6 +354,7 @@ def bench_batched(args, capi, W):
            "batched_vs_plain": t_plain / t_b,
            "results_identical_batched_vs_plain": bool(same),
            "generate_ms": t_gen * 1e3,
+           "tuning": os.path.relpath(tuning, ROOT) if tuning else None,
            "kernels_ms_per_step": {k: v[1] / steps for k, v in sorted(kern.items())},
            "roofline": None}
     if not args.no_cpu:
@@ -427,7 +433,40 @@ def main():
         print(json.dumps(out))
         return
 
-    if args.workload == "saxpy3-er20":
+    if args.workload == "saxpy3-tricount-rmat22":
+        # the triangle count by saxpy3: C<L> = L*L (plus_pair int64, structural
+        # mask), the same count as the dot3 form C<L> = L*U'
+        Ag, L, U = W.tricount_inputs(22, 16, seed=1)
+        Ld = capi.Matrix.from_csr(L, "bool")
+        C = capi.Matrix("int64", L.nrows, L.nrows)
+
+        def step():
+            C.clear()
+            capi.mxm_checked(C, Ld, "GxB_PLUS_PAIR_INT64", Ld, Ld, "S")
+
+        step()
+        ntri = capi.reduce_sum_int64(C)
+        nnzC = C.nvals
+        balg, flops = saxpy3_alg_bytes(L, L, nnzC, sa=0, sb=0, sc=8)
+        ms, kern = timed_steps(capi, step, args.steps, args.warmup)
+        dom = "saxpy3 (all kernels)"
+        kern[dom] = (args.steps, sum(v[1] for k, v in kern.items() if k.startswith("sx_")))
+        mults = int(flops)
+        out.update({"dtype": "int64",
+                    "config": {"workload": "saxpy3 C<L>=L*L plus_pair int64 (triangle count), "
+                                           "RMAT scale 22, ef 16, seeded",
+                               "n": L.nrows, "nnz_L": L.nnz, "triangles": int(ntri),
+                               "nnz_C": nnzC, "flops": mults,
+                               "method": "saxpy3 (masked, hash per row)"}})
+        nr = L.nrows // 64
+        Ls = W.Csr(nr, L.ncols, L.p[:nr + 1].copy(), L.j[:L.p[nr]].copy())
+        cpu_unit = int(saxpy3_alg_bytes(Ls, L, 0)[1])
+        cpu_sample = "rows 0..n/64 of C<L>=L*L (bounded sample)"
+
+        def cpu_fn():
+            import pyoracle as O
+            O.saxpy3("plus_pair_int64", Ls, L, M=Ls)
+    elif args.workload == "saxpy3-er20":
         n = 1 << 20
         A, B = W.er(n, n, 8, seed=1), W.er(n, n, 8, seed=2)
         Ad, Bd = capi.Matrix.from_csr(A, "fp64"), capi.Matrix.from_csr(B, "fp64")

@@ -697,8 +697,7 @@ __global__ void k_sx_iso_one(void *x, int size)
 // only waits on tiles already running; a bounded spin turns a lost
 // predecessor into an error flag instead of a hang.
 
-constexpr int FUSED_WAVES = SX_NT / 64;       // independent wavefronts per workgroup
-constexpr int FUSED_ROWS = 64 / TG;           // rows of a tile: one wavefront's 8 teams
+constexpr int FUSED_ROWS = SX_NT / TG;
 constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
@@ -711,34 +710,29 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                                                     int *err)
 {
     using Z = typename S::Z;
-    constexpr int NTEAM = SX_NT / TG;
     // per-team rows padded by one element so the 8 teams of a wavefront
     // start on different LDS banks
-    __shared__ uint64_t s_key[NTEAM][FMAX + 1];  // sort keys, then output columns (int32)
-    __shared__ Z s_val[NTEAM][FMAX + 1];         // flop values, then output values
-    __shared__ int64_t s_pre[NTEAM][TG + 1], s_k[NTEAM][TG + 1], s_bs[NTEAM][TG + 1];
-    __shared__ Z s_av[NTEAM][TG + 1];
-    __shared__ int64_t s_cnt[NTEAM];
-    __shared__ int64_t s_off[FUSED_WAVES][FUSED_ROWS + 1];
-    __shared__ int s_big[NTEAM];
+    __shared__ uint64_t s_key[FUSED_ROWS][FMAX + 1];  // sort keys, then output columns (int32)
+    __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
+    __shared__ int64_t s_pre[FUSED_ROWS][TG + 1], s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
+    __shared__ Z s_av[FUSED_ROWS][TG + 1];
+    __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
+    __shared__ int s_big[FUSED_ROWS];
+    __shared__ int64_t s_tile, s_prefix;
 
-    // every wavefront works on its own tiles: no workgroup barriers
     const int team = threadIdx.x / TG, sub = threadIdx.x & (TG - 1);
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tw = team & (FUSED_ROWS - 1);       // team within the wavefront
     uint64_t *key = s_key[team];
     Z *val = s_val[team];
     int32_t *outj = (int32_t *) s_key[team];
-    int64_t *off = s_off[wv];
     unsigned long long *ticket = status + ntiles;
 
     while (true)
     {
-        unsigned long long tk = 0;
-        if (lane == 0) tk = atomicAdd(ticket, 1ull);
-        const int64_t tile = (int64_t) __shfl(tk, 0, 64);
+        if (threadIdx.x == 0) s_tile = (int64_t) atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const int64_t tile = s_tile;
         if (tile >= ntiles) break;
-        const int64_t i = tile * FUSED_ROWS + tw;
+        const int64_t i = tile * FUSED_ROWS + team;
 
         #pragma unroll
         for (int u = 0; u < FMAX / TG; u++) key[sub * (FMAX / TG) + u] = ~0ull;
@@ -903,101 +897,105 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
             s_cnt[team] = cnt;
             s_big[team] = big;
         }
-        wave_sync();
-        // tile offsets of the wavefront's rows, then a look-back over up to
-        // LB_WIN x 64 predecessors per step (lane l reads tile - 1 - u*64 - l)
-        const int tbase = wv * FUSED_ROWS;
-        int64_t run = 0;
-        #pragma unroll
-        for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[tbase + t];
-        if (lane == 0)
+        __syncthreads();
+        if (threadIdx.x < 64)
         {
-            int64_t r2 = 0;
-            for (int t = 0; t < FUSED_ROWS; t++) { off[t] = r2; r2 += s_cnt[tbase + t]; }
-            off[FUSED_ROWS] = r2;
-            __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        int64_t prefix = 0;
-        if (tile > 0)
-        {
-            // window positions u*64 + lane = distance - 1; all loads are
-            // independent (relaxed: the counts travel inside the status words)
-            int64_t end = tile - 1;
-            int spins = 0;
-            while (true)
-            {
-                unsigned long long w[LB_WIN];
-                #pragma unroll
-                for (int u = 0; u < LB_WIN; u++)
-                {
-                    const int64_t idx = end - (int64_t) u * 64 - lane;
-                    w[u] = (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT)
-                                      : LB_PRE;
-                }
-                int firstp = LB_WIN * 64;
-                bool zero_before = false;
-                #pragma unroll
-                for (int u = 0; u < LB_WIN; u++)
-                {
-                    const unsigned long long flag = w[u] & ~LB_MASK;
-                    const unsigned long long bp = __ballot(flag == LB_PRE);
-                    const unsigned long long b0 = __ballot(flag == 0);
-                    if (firstp == LB_WIN * 64)
-                    {
-                        const int fp = bp ? (__ffsll((long long) bp) - 1) : 64;
-                        const unsigned long long upto =
-                            (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
-                        if (b0 & upto) zero_before = true;
-                        if (fp < 64) firstp = u * 64 + fp;
-                    }
-                }
-                if (zero_before)
-                {
-                    // a predecessor has not published yet
-                    if (++spins > LB_SPIN_LIMIT)
-                    {
-                        if (lane == 0) atomicExch(err, 1);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                int64_t v = 0;
-                #pragma unroll
-                for (int u = 0; u < LB_WIN; u++)
-                    if (u * 64 + lane <= firstp) v += (int64_t) (w[u] & LB_MASK);
-                #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-                prefix += v;
-                if (firstp < LB_WIN * 64) break;
-                end -= LB_WIN * 64;
-            }
+            // wavefront 0: tile offsets, then a look-back over up to 64
+            // predecessors per step (lane l reads tile - 1 - l)
+            const int lane = threadIdx.x;
+            int64_t run = 0;
+            for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[t];
             if (lane == 0)
-                __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
+            {
+                int64_t r2 = 0;
+                for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = r2; r2 += s_cnt[t]; }
+                s_off[FUSED_ROWS] = r2;
+                __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            int64_t prefix = 0;
+            if (tile > 0)
+            {
+                // window of LB_WIN*64 predecessors, position u*64 + lane =
+                // distance - 1; all loads are independent (relaxed: the
+                // counts travel inside the status words themselves)
+                int64_t end = tile - 1;
+                int spins = 0;
+                while (true)
+                {
+                    unsigned long long w[LB_WIN];
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
+                    {
+                        const int64_t idx = end - (int64_t) u * 64 - lane;
+                        w[u] = (idx >= 0) ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : LB_PRE;
+                    }
+                    int firstp = LB_WIN * 64;
+                    bool zero_before = false;
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
+                    {
+                        const unsigned long long flag = w[u] & ~LB_MASK;
+                        const unsigned long long bp = __ballot(flag == LB_PRE);
+                        const unsigned long long b0 = __ballot(flag == 0);
+                        if (firstp == LB_WIN * 64)
+                        {
+                            const int fp = bp ? (__ffsll((long long) bp) - 1) : 64;
+                            const unsigned long long upto =
+                                (fp >= 63) ? ~0ull : ((2ull << fp) - 1ull);
+                            if (b0 & upto) zero_before = true;
+                            if (fp < 64) firstp = u * 64 + fp;
+                        }
+                    }
+                    if (zero_before)
+                    {
+                        // a predecessor has not published yet
+                        if (++spins > LB_SPIN_LIMIT)
+                        {
+                            if (lane == 0) atomicExch(err, 1);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    int64_t v = 0;
+                    #pragma unroll
+                    for (int u = 0; u < LB_WIN; u++)
+                        if (u * 64 + lane <= firstp) v += (int64_t) (w[u] & LB_MASK);
+                    #pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                    prefix += v;
+                    if (firstp < LB_WIN * 64) break;
+                    end -= LB_WIN * 64;
+                }
+                if (lane == 0)
+                    __hip_atomic_store(&status[tile], LB_PRE | (unsigned long long) (prefix + run),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (lane == 0) s_prefix = prefix;
         }
-        wave_sync();
-        const int64_t total = off[FUSED_ROWS];
-        if (sub == 0 && i < g.m) g.Cp[i] = prefix + off[tw];
-        if (tile == ntiles - 1 && lane == 0) g.Cp[g.m] = prefix + total;
+        __syncthreads();
+        const int64_t prefix = s_prefix;
+        const int64_t total = s_off[FUSED_ROWS];
+        if (sub == 0 && i < g.m) g.Cp[i] = prefix + s_off[team];
+        if (tile == ntiles - 1 && threadIdx.x == 0) g.Cp[g.m] = prefix + total;
         // contiguous copy of the tile's small-row output
-        for (int64_t e = lane; e < total; e += 64)
+        for (int64_t e = threadIdx.x; e < total; e += SX_NT)
         {
             int lo = 0, hi = FUSED_ROWS - 1;
             while (lo < hi)
             {
                 const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= e) lo = mid; else hi = mid - 1;
+                if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
             }
-            if (s_big[tbase + lo]) continue;
-            const int pos = (int) (e - off[lo]);
-            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[tbase + lo])[pos];
-            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[tbase + lo][pos];
+            if (s_big[lo]) continue;
+            const int pos = (int) (e - s_off[lo]);
+            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[lo])[pos];
+            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[lo][pos];
         }
-        // the next tile of this wavefront reuses its LDS rows
-        wave_sync();
+        __syncthreads();
     }
 }
 

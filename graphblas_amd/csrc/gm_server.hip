@@ -62,6 +62,7 @@ struct SvRequest
     int rc = 0;                       // result of the batch's saxpy3 call
     std::shared_ptr<SvBatch> batch;
     int64_t row0 = 0;
+    int64_t nvals = 0;                // entries of the request's rows
 };
 
 } // namespace gm
@@ -164,6 +165,14 @@ static void sv_worker(gm_server *s)
             V.ncols = s->graph.ncols;
             rc = gm_saxpy3(&res->C, &V, true, true, &F, &s->graph, &s->sr, nullptr);
         }
+        // the batch's row pointers, once, so that waiting for a request needs
+        // no device access
+        std::vector<int64_t> hcp;
+        if (rc == GM_SUCCESS)
+        {
+            hcp.resize(K + 1);
+            rc = gm_memcpy_d2h(hcp.data(), res->C.p, sizeof(int64_t) * (K + 1));
+        }
         const double t1 = now_us();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -175,7 +184,11 @@ static void sv_worker(gm_server *s)
                 r->t_done = t1;
                 r->state = 2;
                 r->rc = rc;
-                if (rc == GM_SUCCESS) r->batch = res;
+                if (rc == GM_SUCCESS)
+                {
+                    r->batch = res;
+                    r->nvals = hcp[r->row0 + (int64_t) r->src.size()] - hcp[r->row0];
+                }
                 s->flops_done += (double) r->flops;
             }
             s->completed += (int64_t) batch.size();
@@ -284,17 +297,7 @@ int gm_server_wait(gm_server *s, int64_t id, gm_server_result *out)
     out->queue_us = r->t_start - r->t_arrive;
     out->batch_id = r->batch ? r->batch->id : 0;
     out->batch_size = r->batch ? r->batch->size : 0;
-    if (r->rc == GM_SUCCESS)
-    {
-        // nnz of the request's rows: read its two row pointers
-        int64_t pp[2] = {0, 0};
-        const int64_t k = (int64_t) r->src.size();
-        lk.unlock();
-        int rc = gm_memcpy_d2h(&pp[0], r->batch->C.p + r->row0, sizeof(int64_t));
-        if (rc == GM_SUCCESS) rc = gm_memcpy_d2h(&pp[1], r->batch->C.p + r->row0 + k, sizeof(int64_t));
-        if (rc != GM_SUCCESS) return rc;
-        out->nvals = pp[1] - pp[0];
-    }
+    out->nvals = r->nvals;
     return out->status;
 }
 
