@@ -388,11 +388,12 @@ __device__ void dot_lane(const Dot3Args &g, const Pair &P, int lane, bool &e, ty
     }
 }
 
-// LDS words per team for staged pairs: both index lists (as 32-bit columns)
+// LDS words per team for staged pairs (16 KB per workgroup, so LDS does not
+// cap the occupancy): both index lists (as 32-bit columns)
 // of a merge/search pair with a+b <= cap are copied to LDS by the team with
 // coalesced loads, then intersected there, so the merge walk and the binary
 // searches pay LDS latency instead of a chain of dependent global loads.
-template <int G> struct StageCap { static constexpr int value = G == 16 ? 512 : 2048; };
+template <int G> struct StageCap { static constexpr int value = G == 16 ? 256 : 1024; };
 
 // make the team's LDS stores visible to the team's later loads (the team is
 // part of one wavefront, so no workgroup barrier is needed)
@@ -571,8 +572,7 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                 if (take)
                 {
                     g.bins[(int64_t) b * g.mnz + (int64_t) slot] = ((uint64_t) i << 32) | (uint64_t) p;
-                    // keys only for the two team bins (bkeys holds 2 x mnz)
-                    if (g.bkeys && b < 2) g.bkeys[(int64_t) b * g.mnz + (int64_t) slot] = key;
+                    if (g.bkeys) g.bkeys[(int64_t) b * g.mnz + (int64_t) slot] = key;
                 }
             }
         }
@@ -655,7 +655,32 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_team_grp(Dot3Args g, const uint6
     }
 }
 
-// one 256-thread workgroup per entry (the heaviest bin)
+// one deferred entry computed by a whole 256-thread workgroup (the heaviest bin)
+template <class S>
+__device__ __forceinline__ void block_task(const Dot3Args &g, uint64_t w, typename S::Z *s_z, int *s_e)
+{
+    using Z = typename S::Z;
+    const int wid = threadIdx.x >> 6;
+    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+    const int64_t j = g.Mj[p];
+    bool e;
+    Z z;
+    Pair P;
+    (void) d3_pair(g, i, j, P);
+    dot_lane<S, D3_NT>(g, P, threadIdx.x, e, z);
+    team_reduce<S, 64>(e, z);
+    if ((threadIdx.x & 63) == 0) { s_z[wid] = z; s_e[wid] = e ? 1 : 0; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        bool ee = s_e[0] != 0;
+        Z zz = s_z[0];
+        for (int t = 1; t < D3_NT / 64; t++) combine<S>(ee, zz, s_e[t] != 0, s_z[t]);
+        d3_store<S>(g, p, ee, zz);
+    }
+    __syncthreads();
+}
+
 template <class S>
 __global__ __launch_bounds__(D3_NT) void k_dot3_block(Dot3Args g)
 {
@@ -664,28 +689,33 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block(Dot3Args g)
     __shared__ int s_e[D3_NT / 64];
     const int64_t count = (int64_t) g.bin_cnt[2];
     const uint64_t *list = g.bins + 2 * g.mnz;
-    const int wid = threadIdx.x >> 6;
-    for (int64_t q = blockIdx.x; q < count; q += gridDim.x)
+    for (int64_t q = blockIdx.x; q < count; q += gridDim.x) block_task<S>(g, list[q], s_z, s_e);
+}
+
+// the heaviest bin sorted by probed row, over the same 8 XCD regions as the
+// grouped team kernel (one entry per chunk)
+template <class S>
+__global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint64_t *list,
+                                                          int64_t count, unsigned long long *ctr)
+{
+    using Z = typename S::Z;
+    __shared__ Z s_z[D3_NT / 64];
+    __shared__ int s_e[D3_NT / 64];
+    __shared__ int64_t s_q;
+    const int x0 = (int) (blockIdx.x & 7);
+    for (int k = 0; k < 8; k++)
     {
-        const uint64_t w = list[q];
-        const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
-        const int64_t j = g.Mj[p];
-        bool e;
-        Z z;
-        Pair P;
-        (void) d3_pair(g, i, j, P);
-        dot_lane<S, D3_NT>(g, P, threadIdx.x, e, z);
-        team_reduce<S, 64>(e, z);
-        if ((threadIdx.x & 63) == 0) { s_z[wid] = z; s_e[wid] = e ? 1 : 0; }
-        __syncthreads();
-        if (threadIdx.x == 0)
+        const int r = (x0 + k) & 7;
+        const int64_t lo = count * r / 8, hi = count * (r + 1) / 8;
+        while (true)
         {
-            bool ee = s_e[0] != 0;
-            Z zz = s_z[0];
-            for (int t = 1; t < D3_NT / 64; t++) combine<S>(ee, zz, s_e[t] != 0, s_z[t]);
-            d3_store<S>(g, p, ee, zz);
+            if (threadIdx.x == 0) s_q = lo + (int64_t) atomicAdd(&ctr[r], 1ull);
+            __syncthreads();
+            const int64_t q = s_q;
+            __syncthreads();
+            if (q >= hi) break;
+            block_task<S>(g, list[q], s_z, s_e);
         }
-        __syncthreads();
     }
 }
 
@@ -998,8 +1028,8 @@ static int dot3_bitmaps(Dot3Args &g, const gm_csr *A, const gm_csr *B, bool need
     return GM_SUCCESS;
 }
 
-// Sort the team16 and wave bins by their grouping keys (radix sort of
-// (key, entry) pairs) and run them with the XCD-regioned team kernel.  The
+// Sort the team16, wave and workgroup bins by their grouping keys (radix sort
+// of (key, entry) pairs) and run them with the XCD-regioned kernels.  The
 // bins' keys must have been written by phase1 (g.bkeys != NULL).
 template <class S>
 static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
@@ -1007,20 +1037,21 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
     hipStream_t st = stream();
     unsigned long long *hc = (unsigned long long *) pinned_scratch(4 * sizeof(unsigned long long));
     if (!hc) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
-    GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+    GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
            "bin counts");
     GM_HIP(hipStreamSynchronize(st), "bin counts sync");
-    const int64_t cnt[2] = {(int64_t) hc[0], (int64_t) hc[1]};
-    const int64_t big = cnt[0] > cnt[1] ? cnt[0] : cnt[1];
+    const int64_t cnt[3] = {(int64_t) hc[0], (int64_t) hc[1], (int64_t) hc[2]};
+    int64_t big = cnt[0] > cnt[1] ? cnt[0] : cnt[1];
+    if (cnt[2] > big) big = cnt[2];
     if (big == 0) return GM_SUCCESS;
     int end_bit = 1;
     while (end_bit < 64 && ((int64_t) 1 << end_bit) <= nkeys) end_bit++;
     DevBuf tk, tv, tmp, ctr;
     if (!tk.alloc(sizeof(uint64_t) * big) || !tv.alloc(sizeof(uint64_t) * big) ||
-        !ctr.alloc(16 * sizeof(unsigned long long)))
+        !ctr.alloc(24 * sizeof(unsigned long long)))
         return set_error(GM_OUT_OF_MEMORY, "dot3: grouping buffers");
     size_t tbytes = 0;
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 3; b++)
     {
         size_t tb = 0;
         GM_HIP(rocprim::radix_sort_pairs(nullptr, tb, (const uint64_t *) g.bkeys, tk.as<uint64_t>(),
@@ -1029,8 +1060,8 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
         if (tb > tbytes) tbytes = tb;
     }
     if (!tmp.alloc(tbytes)) return set_error(GM_OUT_OF_MEMORY, "dot3: grouping sort");
-    k_zero_u64<<<1, 64, 0, st>>>(ctr.as<unsigned long long>(), 16);
-    for (int b = 0; b < 2; b++)
+    k_zero_u64<<<1, 64, 0, st>>>(ctr.as<unsigned long long>(), 24);
+    for (int b = 0; b < 3; b++)
     {
         if (cnt[b] == 0) continue;
         {
@@ -1049,12 +1080,19 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
                 g, tv.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
             GM_HIP(hipGetLastError(), "k_dot3_team_grp16");
         }
-        else
+        else if (b == 1)
         {
             ProfScope ps("dot3_wave");
             k_dot3_team_grp<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
                 g, tv.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
             GM_HIP(hipGetLastError(), "k_dot3_team_grp64");
+        }
+        else
+        {
+            ProfScope ps("dot3_block");
+            k_dot3_block_grp<S><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                g, tv.as<uint64_t>(), cnt[2], ctr.as<unsigned long long>() + 16);
+            GM_HIP(hipGetLastError(), "k_dot3_block_grp");
         }
     }
     return GM_SUCCESS;
@@ -1111,7 +1149,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         DevBuf bkeys;
         if (grouped)
         {
-            if (!bkeys.alloc(sizeof(uint64_t) * 2 * Mnz))
+            if (!bkeys.alloc(sizeof(uint64_t) * 3 * Mnz))
                 return set_error(GM_OUT_OF_MEMORY, "dot3: grouping keys");
             g.bkeys = bkeys.as<uint64_t>();
         }
@@ -1151,6 +1189,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
                 GM_HIP(hipGetLastError(), "k_dot3_wave");
             }
         }
+        if (!grouped)
         {
             ProfScope ps("dot3_block");
             k_dot3_block<S><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g);

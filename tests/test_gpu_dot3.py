@@ -144,7 +144,7 @@ def test_dot3_bitmaps_distinct_operands(grb, handle):
 
 @pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("params", [(1, 2, 100000, -1), (1, 64, 100000, 8), (1, 16, 100000, 1),
-                                    (0, 0, 0, 0)])
+                                    (0, 0, 0, 0), (1, 2, 3, -1), (1, 2, 3, 8)])
 @pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
                                     "GrB_MIN_PLUS_SEMIRING_INT32", "GxB_ANY_SECONDI_INT64"])
 def test_dot3_entry_order(grb, order, params, handle):
