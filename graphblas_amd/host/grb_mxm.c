@@ -22,11 +22,12 @@
 
 GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran, int axb)
 {
-    (void) axb;
     (void) a_tran;      // an A' operand is transposed first; the choice is the same
-    // C<M> = A*B' -> dot3; every other shape (after transposing B' when
-    // needed) -> saxpy3
-    if (b_tran && has_mask && !mask_comp) return GB_METHOD_DOT3;
+    // C<M> = A*B' -> dot3 (GB_AxB_dot3_control, Source/mxm/GB_mxm.h:233-241);
+    // C<M> = A*B with the GxB_AxB_DOT hint -> dot3 on a transposed B (the
+    // user override of GB_AxB_meta_adotb_control.c:88-92); every other shape
+    // (after transposing B' when needed) -> saxpy3
+    if (has_mask && !mask_comp && (b_tran || axb == GxB_AxB_DOT)) return GB_METHOD_DOT3;
     return GB_METHOD_SAXPY3;
 }
 
@@ -155,8 +156,9 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
         va = GB_view_of(&AT);
     }
-    if (b_tran && method == GB_METHOD_SAXPY3)
+    if (b_tran != (method == GB_METHOD_DOT3))
     {
+        // saxpy3 needs B by rows, dot3 needs the rows of B' (B's columns)
         if (GB_burble) printf("burble: GrB_mxm: transpose B (%lld entries)\n", (long long) B->nvals);
         rc = gm_transpose(&BT, &vb);
         if (rc != GM_SUCCESS)

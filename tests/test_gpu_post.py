@@ -180,3 +180,24 @@ def test_grb_mxm_accum_without_mask(grb):
     T = O.saxpy3("plus_times_fp64", A, B)
     ref = O.accum_mask(C0, None, T, op=1, dtype=np.float64)
     assert_csr_equal(C.to_csr(), ref, fp_tol=1e-12, what="C += A*B")
+
+
+def test_grb_mxm_axb_dot_hint(grb):
+    """C<M> = A*B with GxB_AxB_METHOD = GxB_AxB_DOT: B is transposed and dot3
+    runs; the result equals the saxpy3 one."""
+    L = capi.lib()
+    rng = np.random.default_rng(29)
+    A = rand_csr(70, 60, 0.08, rng, "int64")
+    B = rand_csr(60, 50, 0.08, rng, "int64")
+    M = rand_csr(70, 50, 0.2, rng, "int64")
+    d = ctypes.c_void_p()
+    assert L.GrB_Descriptor_new(ctypes.byref(d)) == 0
+    assert L.GrB_Descriptor_set(d, 7090, 7083) == 0          # GxB_AxB_METHOD = GxB_AxB_DOT
+    assert L.GrB_Descriptor_set(d, 1, 4) == 0                # structural mask
+    C = capi.Matrix("int64", 70, 50)
+    rc = L.GrB_mxm(C.h, _mat(M, "int64").h, None, capi.handle("GrB_PLUS_TIMES_SEMIRING_INT64"),
+                   _mat(A, "int64").h, _mat(B, "int64").h, d)
+    L.GrB_Descriptor_free(ctypes.byref(d))
+    assert rc == 0, capi.last_error()
+    ref = O.saxpy3("plus_times_int64", A, B, M=M)
+    assert_csr_equal(C.to_csr(), ref, what="axb dot")

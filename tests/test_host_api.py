@@ -76,6 +76,8 @@ def test_method_selection():
     assert sel(True, True, False, False, 0) == 2       # C<!M>=A*B (BFS, RSC)
     assert sel(True, False, True, False, 0) == 2       # C<M>=A'*B: transpose A, saxpy3
     assert sel(True, False, True, True, 0) == 1        # C<M>=A'*B': transpose A, dot3
+    assert sel(True, False, False, False, 7083) == 1   # C<M>=A*B, GxB_AxB_DOT: transpose B, dot3
+    assert sel(True, True, False, False, 7083) == 2    # C<!M>: dot3 does not take it
 
 
 def test_branch_predicates_host_only():
