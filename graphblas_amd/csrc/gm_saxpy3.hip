@@ -704,7 +704,9 @@ constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 22;
 constexpr int LB_WIN = 16;           // look-back window: LB_WIN x 64 predecessors
 
-template <class S>
+// K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
+// the key LDS and the sort's shuffles and registers.
+template <class S, class K>
 __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                                                     unsigned long long *status, int64_t ntiles,
                                                     int *err)
@@ -712,16 +714,17 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
     using Z = typename S::Z;
     // per-team rows padded by one element so the 8 teams of a wavefront
     // start on different LDS banks
-    __shared__ uint64_t s_key[FUSED_ROWS][FMAX + 1];  // sort keys, then output columns (int32)
+    __shared__ K s_key[FUSED_ROWS][FMAX + 1];         // sort keys, then output columns (int32)
     __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
-    __shared__ int64_t s_pre[FUSED_ROWS][TG + 1], s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
+    __shared__ int32_t s_pre[FUSED_ROWS][TG + 1];
+    __shared__ int64_t s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
     __shared__ Z s_av[FUSED_ROWS][TG + 1];
     __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
     __shared__ int s_big[FUSED_ROWS];
     __shared__ int64_t s_tile, s_prefix;
 
     const int team = threadIdx.x / TG, sub = threadIdx.x & (TG - 1);
-    uint64_t *key = s_key[team];
+    K *key = s_key[team];
     Z *val = s_val[team];
     int32_t *outj = (int32_t *) s_key[team];
     unsigned long long *ticket = status + ntiles;
@@ -735,7 +738,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
         const int64_t i = tile * FUSED_ROWS + team;
 
         #pragma unroll
-        for (int u = 0; u < FMAX / TG; u++) key[sub * (FMAX / TG) + u] = ~0ull;
+        for (int u = 0; u < FMAX / TG; u++) key[sub * (FMAX / TG) + u] = (K) ~(K) 0;
         int64_t f = 0, cnt = 0;
         int big = 0, nallow = 0;
         if (i < g.m)
@@ -758,7 +761,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                 const int64_t inc = team_incl_scan<int64_t>(len);
                 const int64_t tot = __shfl(inc, TG - 1, TG);
                 if (f + tot > fs) { big = 1; break; }
-                s_pre[team][sub] = inc - len;
+                s_pre[team][sub] = (int32_t) (inc - len);
                 s_k[team][sub] = k;
                 s_bs[team][sub] = bs;
                 s_av[team][sub] = av;
@@ -781,7 +784,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                         if (sx_mask_ok(g, pm0, pm1, j))
                         {
                             const int64_t fq = f + q;
-                            key[fq] = ((uint64_t) j << 6) | (uint64_t) fq;
+                            key[fq] = (K) (((K) j << 6) | (K) fq);
                             if constexpr (!S::iso_out)
                                 val[fq] = S::mul(s_av[team][lo], S::geta((const Z *) g.Bx, pb, g.b_iso),
                                                  s_k[team][lo]);
@@ -801,7 +804,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
             // sub*8 .. sub*8+7 in registers; distances < 8 are register
             // compare-exchanges, distances >= 8 are width-8 shuffles
             constexpr int PER_L = FMAX / TG;
-            uint64_t kr[PER_L];
+            K kr[PER_L];
             #pragma unroll
             for (int r = 0; r < PER_L; r++) kr[r] = key[sub * PER_L + r];
             #pragma unroll
@@ -819,8 +822,12 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                         {
                             const int idx = sub * PER_L + r;
                             const bool asc = (idx & kk) == 0;
-                            const uint64_t o = (uint64_t) __shfl_xor((unsigned long long) kr[r], lmask, TG);
-                            const uint64_t mn = kr[r] < o ? kr[r] : o, mx = kr[r] < o ? o : kr[r];
+                            K o;
+                            if constexpr (sizeof(K) == 4)
+                                o = (K) __shfl_xor((unsigned int) kr[r], lmask, TG);
+                            else
+                                o = (K) __shfl_xor((unsigned long long) kr[r], lmask, TG);
+                            const K mn = kr[r] < o ? kr[r] : o, mx = kr[r] < o ? o : kr[r];
                             kr[r] = (lower == asc) ? mn : mx;
                         }
                     }
@@ -834,7 +841,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                             {
                                 const int idx = sub * PER_L + r;
                                 const bool asc = (idx & kk) == 0;
-                                const uint64_t a = kr[r], b = kr[partner];
+                                const K a = kr[r], b = kr[partner];
                                 if ((a > b) == asc) { kr[r] = b; kr[partner] = a; }
                             }
                         }
@@ -856,7 +863,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                 hj[u] = -1;
                 if (p < nallow)
                 {
-                    const uint64_t kp = key[p];
+                    const K kp = key[p];
                     const int64_t jp = (int64_t) (kp >> 6);
                     if (p == 0 || (int64_t) (key[p - 1] >> 6) != jp)
                     {
@@ -866,7 +873,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
                             z = val[kp & 63];
                             for (int p2 = p + 1; p2 < nallow; p2++)
                             {
-                                const uint64_t k2 = key[p2];
+                                const K k2 = key[p2];
                                 if ((int64_t) (k2 >> 6) != jp) break;
                                 z = S::upd(z, val[k2 & 63]);
                             }
@@ -1224,8 +1231,12 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         {
             ProfScope ps("sx_fused");
             const int64_t grid = ntiles < (int64_t) cu_count() * 8 ? ntiles : (int64_t) cu_count() * 8;
-            k_sx_fused<S><<<(unsigned) grid, SX_NT, 0, st>>>(g, fs, status.as<unsigned long long>(),
-                                                             ntiles, (int *) (g.cnt + 12));
+            if (B->ncols <= ((int64_t) 1 << 26))
+                k_sx_fused<S, uint32_t><<<(unsigned) grid, SX_NT, 0, st>>>(
+                    g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
+            else
+                k_sx_fused<S, uint64_t><<<(unsigned) grid, SX_NT, 0, st>>>(
+                    g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
             GM_HIP(hipGetLastError(), "k_sx_fused");
         }
         GM_HIP(hipMemcpyAsync(hp + 11, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");

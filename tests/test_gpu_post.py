@@ -195,8 +195,8 @@ def test_grb_mxm_axb_dot_hint(grb):
     assert L.GrB_Descriptor_set(d, 7090, 7083) == 0          # GxB_AxB_METHOD = GxB_AxB_DOT
     assert L.GrB_Descriptor_set(d, 1, 4) == 0                # structural mask
     C = capi.Matrix("int64", 70, 50)
-    rc = L.GrB_mxm(C.h, _mat(M, "int64").h, None, capi.handle("GrB_PLUS_TIMES_SEMIRING_INT64"),
-                   _mat(A, "int64").h, _mat(B, "int64").h, d)
+    Md, Ad, Bd = _mat(M, "int64"), _mat(A, "int64"), _mat(B, "int64")   # keep the handles alive
+    rc = L.GrB_mxm(C.h, Md.h, None, capi.handle("GrB_PLUS_TIMES_SEMIRING_INT64"), Ad.h, Bd.h, d)
     L.GrB_Descriptor_free(ctypes.byref(d))
     assert rc == 0, capi.last_error()
     ref = O.saxpy3("plus_times_int64", A, B, M=M)
