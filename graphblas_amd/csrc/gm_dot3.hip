@@ -242,6 +242,51 @@ __device__ __forceinline__ void d3_trim_short(const Dot3Args &g, const Pair &P, 
     q1 = s1;
 }
 
+// First index in [lo, hi) with X[idx] >= v (hi if none), found by the G lanes
+// of a team together (G <= 64, one wavefront): each round the lanes sample G
+// evenly spaced positions and a ballot narrows the range G-fold, so the
+// search costs log_G(hi-lo) rounds of independent loads instead of
+// log_2(hi-lo) dependent ones.  Every lane of the team must call it.
+template <int G>
+__device__ __forceinline__ int64_t team_lower_bound(const int64_t *X, int64_t lo, int64_t hi, int64_t v,
+                                                    int lane)
+{
+    static_assert(G <= 64, "one wavefront");
+    const int base = (int) (threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long tmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    while (hi > lo)
+    {
+        const int64_t len = hi - lo;
+        const int64_t stride = (len + G - 1) / G;
+        const int64_t pos = lo + (int64_t) lane * stride;
+        const bool less = pos < hi && X[pos] < v;
+        const int cnt = __popcll((__ballot(less) >> base) & tmask);
+        if (stride == 1) return lo + cnt;
+        if (cnt == 0) return lo;
+        const int64_t nlo = lo + (int64_t) (cnt - 1) * stride + 1;
+        const int64_t nhi = lo + (int64_t) cnt * stride;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+    }
+    return lo;
+}
+
+// d3_trim_short with the two searches done by the team
+template <int G>
+__device__ __forceinline__ void d3_trim_short_team(const Dot3Args &g, const Pair &P, int lane,
+                                                   int64_t &q0, int64_t &q1)
+{
+    const bool shortA = (P.probe == 2);
+    const int64_t *Sj = shortA ? g.Aj : g.Bj;
+    int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
+    const int64_t slo = Sj[s0], shi = Sj[s1 - 1];
+    const int64_t olo = slo > P.lo ? slo : P.lo, ohi = shi < P.hi ? shi : P.hi;
+    if (olo > slo) s0 = team_lower_bound<G>(Sj, s0, s1, olo, lane);
+    if (ohi < shi) s1 = team_lower_bound<G>(Sj, s0, s1, ohi + 1, lane);
+    q0 = s0;
+    q1 = s1;
+}
+
 // Single-thread dot product: bitmap probes, or a linear merge / binary trim
 // of the longer list when it is much longer (the reference's rule at
 // GB_AxB_dot_cij.c:538-616, here chosen by estimated cost).
@@ -317,7 +362,8 @@ __device__ void dot_lane(const Dot3Args &g, const Pair &P, int lane, bool &e, ty
         // the short row, trimmed to the overlap of both ranges, interleaved
         // over the lanes (coalesced reads), each probed in the long row's index
         int64_t q0, q1;
-        d3_trim_short(g, P, q0, q1);
+        if constexpr (G <= 64) d3_trim_short_team<G>(g, P, lane, q0, q1);
+        else d3_trim_short(g, P, q0, q1);
         d3_probe_range<S>(g, P, q0 + lane, q1, G, e, z);
         return;
     }
