@@ -321,10 +321,14 @@ def bench_batched(args, capi, W):
         for ptr in (out.p, out.j, out.x):
             L.gm_free(ptr)
     L.gm_synchronize()
+    capi.prof_reset()
+    capi.prof_enable(True)
     t0 = time.perf_counter()
     plain(False)
     L.gm_synchronize()
     t_plain = time.perf_counter() - t0
+    capi.prof_enable(False)
+    plain_kernel_ms = sum(v[1] for v in capi.prof_get().values())
     steps = max(1, args.steps)
     capi.prof_reset()
     capi.prof_enable(True)
@@ -349,8 +353,8 @@ def bench_batched(args, capi, W):
                                 "semiring) vs gm_dot3 per request"},
            "value": mults / t_b, "ms_per_step": t_b * 1e3, "steps": steps,
            "total_mults": int(mults),
-           "plain": {"ms": t_plain * 1e3, "mults_per_s": mults / t_plain,
-                     "requests_per_s": nreq / t_plain},
+           "plain": {"ms": t_plain * 1e3, "kernel_ms": plain_kernel_ms,
+                     "mults_per_s": mults / t_plain, "requests_per_s": nreq / t_plain},
            "batched_vs_plain": t_plain / t_b,
            "results_identical_batched_vs_plain": bool(same),
            "generate_ms": t_gen * 1e3,
