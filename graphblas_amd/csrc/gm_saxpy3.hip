@@ -697,7 +697,8 @@ __global__ void k_sx_iso_one(void *x, int size)
 // only waits on tiles already running; a bounded spin turns a lost
 // predecessor into an error flag instead of a hang.
 
-constexpr int FUSED_ROWS = SX_NT / TG;
+constexpr int FUSED_NT = 256;              // threads of a fused-kernel workgroup
+constexpr int FUSED_ROWS = FUSED_NT / TG;  // rows of a tile
 constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
@@ -706,12 +707,13 @@ constexpr int LB_WIN = 16;           // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
-template <class S, class K>
-__global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
-                                                    unsigned long long *status, int64_t ntiles,
-                                                    int *err)
+template <class S, class K, int NT>
+__global__ __launch_bounds__(NT, 768 / NT) void k_sx_fused(SxArgs g, int64_t fs,
+                                                         unsigned long long *status, int64_t ntiles,
+                                                         int *err)
 {
     using Z = typename S::Z;
+    constexpr int FUSED_ROWS = NT / TG;
     // per-team rows padded by one element so the 8 teams of a wavefront
     // start on different LDS banks
     __shared__ K s_key[FUSED_ROWS][FMAX + 1];         // sort keys, then output columns (int32)
@@ -989,7 +991,7 @@ __global__ __launch_bounds__(SX_NT, 3) void k_sx_fused(SxArgs g, int64_t fs,
         if (sub == 0 && i < g.m) g.Cp[i] = prefix + s_off[team];
         if (tile == ntiles - 1 && threadIdx.x == 0) g.Cp[g.m] = prefix + total;
         // contiguous copy of the tile's small-row output
-        for (int64_t e = threadIdx.x; e < total; e += SX_NT)
+        for (int64_t e = threadIdx.x; e < total; e += NT)
         {
             int lo = 0, hi = FUSED_ROWS - 1;
             while (lo < hi)
@@ -1230,12 +1232,13 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         GM_HIP(hipGetLastError(), "k_sx_fill_u64");
         {
             ProfScope ps("sx_fused");
-            const int64_t grid = ntiles < (int64_t) cu_count() * 8 ? ntiles : (int64_t) cu_count() * 8;
+            const int64_t gcap = (int64_t) cu_count() * 8 * (256 / FUSED_NT);
+            const int64_t grid = ntiles < gcap ? ntiles : gcap;
             if (B->ncols <= ((int64_t) 1 << 26))
-                k_sx_fused<S, uint32_t><<<(unsigned) grid, SX_NT, 0, st>>>(
+                k_sx_fused<S, uint32_t, FUSED_NT><<<(unsigned) grid, FUSED_NT, 0, st>>>(
                     g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
             else
-                k_sx_fused<S, uint64_t><<<(unsigned) grid, SX_NT, 0, st>>>(
+                k_sx_fused<S, uint64_t, FUSED_NT><<<(unsigned) grid, FUSED_NT, 0, st>>>(
                     g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
             GM_HIP(hipGetLastError(), "k_sx_fused");
         }
