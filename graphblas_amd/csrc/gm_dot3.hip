@@ -1199,7 +1199,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
                 return set_error(GM_OUT_OF_MEMORY, "dot3: grouping keys");
             g.bkeys = bkeys.as<uint64_t>();
         }
-        g.thread_max = pick_param(prm ? prm->thread_max : 0, tu.thread_max, 32);
+        g.thread_max = pick_param(prm ? prm->thread_max : 0, tu.thread_max, 64);
         g.team16_max = pick_param(prm ? prm->team16_max : 0, tu.team16_max, 512);
         g.wave_max = pick_param(prm ? prm->wave_max : 0, tu.wave_max, 4096);
         const int64_t cap = (int64_t) cu_count() * 16;
@@ -1207,7 +1207,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         g.bB = g.bA;
         DevBuf bm_offA, bm_offB, bm_words, bm_rank, bm_list, bm_ctr;
         int64_t minlen = (prm && prm->bitmap_min_len != 0) ? prm->bitmap_min_len
-                       : (tu.bitmap_min_len != 0 ? tu.bitmap_min_len : 256);
+                       : (tu.bitmap_min_len != 0 ? tu.bitmap_min_len : 128);
         if (A->ncols >= ((int64_t) 1 << 32)) minlen = -1;   // hash keys are 32-bit columns
         if (minlen > 0)
             GM_TRY(dot3_bitmaps(g, A, B, !S::pattern, minlen, bm_offA, bm_offB, bm_words, bm_rank,

@@ -31,7 +31,7 @@ DOT3_SPACE = {
     "bitmap_min_len": [64, 128, 256, 512, -1],
     "order": [1, 2],
 }
-DOT3_BUILTIN = {"thread_max": 32, "team16_max": 512, "wave_max": 4096, "bitmap_min_len": 256,
+DOT3_BUILTIN = {"thread_max": 64, "team16_max": 512, "wave_max": 4096, "bitmap_min_len": 128,
                 "order": 0}
 SAXPY3_SPACE = {
     "wave_small_max": [16, 32, 64],

@@ -104,11 +104,11 @@ typedef struct
 // min(|A(i,:)|+|B(j,:)|, s*ceil(log2(l+1))) with s,l the short/long list.
 typedef struct
 {
-    int64_t thread_max;   // work <= thread_max: one thread per entry (32)
+    int64_t thread_max;   // work <= thread_max: one thread per entry (64)
     int64_t team16_max;   // work <= team16_max: 16 lanes per entry (512)
     int64_t wave_max;     // work <= wave_max: one wavefront per entry (4096)
                           // larger: one 256-thread workgroup per entry
-    int64_t bitmap_min_len;  // rows at least this long may get a bitmap (256);
+    int64_t bitmap_min_len;  // rows at least this long may get a bitmap (128);
                              // < 0: no bitmaps (merge / search only)
     int64_t order;        // order of the team16 / wave entries: 0 auto (grouped
                           // when nnz(M) >= 2^20), 1 mask order, 2 grouped by
