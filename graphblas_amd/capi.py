@@ -233,6 +233,11 @@ def lib():
                             ctypes.c_int),
         "gm_batch_split": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsrOut), P, P, I64],
                            ctypes.c_int),
+        "GxB_Matrix_serialize": ([PP, ctypes.POINTER(U64), P, P], ctypes.c_int),
+        "GrB_Matrix_serialize": ([P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_serializeSize": ([ctypes.POINTER(U64), P], ctypes.c_int),
+        "GxB_Matrix_deserialize": ([PP, P, P, U64, P], ctypes.c_int),
+        "GrB_Matrix_deserialize": ([PP, P, P, U64], ctypes.c_int),
         "GrB_Vector_new": ([PP, P, U64], ctypes.c_int),
         "GrB_Vector_free": ([PP], ctypes.c_int),
         "GrB_Vector_nvals": ([ctypes.POINTER(U64), P], ctypes.c_int),

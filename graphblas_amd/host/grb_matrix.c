@@ -280,15 +280,17 @@ static int GB_tuple_cmp(const void *a, const void *b)
 // z = dup(x, y) for the built-in operators; false if not applicable
 static bool GB_apply_dup(const GrB_BinaryOp op, int32_t code, void *z, const void *x, const void *y)
 {
-#define GB_DUP_CASE(CODE, CT)                                            \
+    // integer arithmetic wraps (computed in the unsigned type); bool PLUS,
+    // MINUS and TIMES are OR, XOR and AND
+#define GB_DUP_CASE(CODE, CT, UT, ADD, SUB, MUL) /* UT: the type ADD/SUB/MUL use */                         \
     if (code == CODE)                                                    \
     {                                                                    \
         CT a = *(const CT *) x, b = *(const CT *) y, r;                  \
         switch (op->opcode)                                              \
         {                                                                \
-            case GB_OP_PLUS: r = (CT) (a + b); break;                    \
-            case GB_OP_MINUS: r = (CT) (a - b); break;                   \
-            case GB_OP_TIMES: r = (CT) (a * b); break;                   \
+            case GB_OP_PLUS: r = (CT) (ADD); break;                      \
+            case GB_OP_MINUS: r = (CT) (SUB); break;                     \
+            case GB_OP_TIMES: r = (CT) (MUL); break;                     \
             case GB_OP_MIN: r = a < b ? a : b; break;                    \
             case GB_OP_MAX: r = a > b ? a : b; break;                    \
             case GB_OP_FIRST: case GB_OP_ANY: r = a; break;              \
@@ -301,10 +303,12 @@ static bool GB_apply_dup(const GrB_BinaryOp op, int32_t code, void *z, const voi
         *(CT *) z = r;                                                   \
         return true;                                                     \
     }
-    GB_DUP_CASE(GM_BOOL, bool)
-    GB_DUP_CASE(GM_INT32, int32_t)
-    GB_DUP_CASE(GM_INT64, int64_t)
-    GB_DUP_CASE(GM_FP64, double)
+    GB_DUP_CASE(GM_BOOL, bool, bool, a || b, a != b, a && b)
+    GB_DUP_CASE(GM_INT32, int32_t, uint32_t, (uint32_t) a + (uint32_t) b, (uint32_t) a - (uint32_t) b,
+                (uint32_t) a * (uint32_t) b)
+    GB_DUP_CASE(GM_INT64, int64_t, uint64_t, (uint64_t) a + (uint64_t) b, (uint64_t) a - (uint64_t) b,
+                (uint64_t) a * (uint64_t) b)
+    GB_DUP_CASE(GM_FP64, double, double, a + b, a - b, a * b)
 #undef GB_DUP_CASE
     return false;
 }
@@ -552,4 +556,154 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
     if (jumbled) *jumbled = false;
     GB_matrix_free_content(A);
     return GrB_SUCCESS;
+}
+
+//------------------------------------------------------------------------------
+// serialize / deserialize (Include/GraphBLAS.h:10917-10990 of the reference)
+//------------------------------------------------------------------------------
+
+// Blob layout (this library's own wire format, little-endian, uncompressed):
+//   header  8 x uint64: magic "GMCSR001", type code, iso, nrows, ncols, nvals,
+//           value size, total blob size
+//   p       (nrows+1) int64 row pointers
+//   j       nvals int64 column indices (rows sorted)
+//   x       nvals values (1 if iso) of the value size
+// A blob from the reference's own format is rejected with
+// GrB_INVALID_OBJECT (its header does not carry this magic).
+
+#define GB_BLOB_MAGIC 0x313030525343474dULL     // "GMCSR001"
+#define GB_BLOB_HDR 8
+
+static GrB_Index GB_blob_size(int64_t nrows, int64_t nvals, size_t xsize, bool iso)
+{
+    return (GrB_Index) (sizeof(uint64_t) * GB_BLOB_HDR + sizeof(int64_t) * (size_t) (nrows + 1) +
+                        sizeof(int64_t) * (size_t) nvals + xsize * (size_t) (iso ? 1 : nvals));
+}
+
+GrB_Info GrB_Matrix_serializeSize(GrB_Index *blob_size_handle, GrB_Matrix A)
+{
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!blob_size_handle) return GrB_NULL_POINTER;
+    *blob_size_handle = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_serialize(void *blob, GrB_Index *blob_size_handle, GrB_Matrix A)
+{
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!blob || !blob_size_handle) return GrB_NULL_POINTER;
+    const GrB_Index need = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    if (*blob_size_handle < need)
+        return GB_error(A, GrB_INSUFFICIENT_SPACE, "serialize: blob of %llu bytes, %llu needed",
+                        (unsigned long long) *blob_size_handle, (unsigned long long) need);
+    int64_t *hp, *hj;
+    void *hx;
+    if ((info = GB_download(A, &hp, &hj, &hx)) != GrB_SUCCESS) return info;
+    uint64_t *h = (uint64_t *) blob;
+    h[0] = GB_BLOB_MAGIC;
+    h[1] = (uint64_t) A->type->code;
+    h[2] = A->iso ? 1 : 0;
+    h[3] = (uint64_t) A->nrows;
+    h[4] = (uint64_t) A->ncols;
+    h[5] = (uint64_t) A->nvals;
+    h[6] = (uint64_t) A->type->size;
+    h[7] = (uint64_t) need;
+    uint8_t *q = (uint8_t *) blob + sizeof(uint64_t) * GB_BLOB_HDR;
+    memcpy(q, hp, sizeof(int64_t) * (size_t) (A->nrows + 1));
+    q += sizeof(int64_t) * (size_t) (A->nrows + 1);
+    memcpy(q, hj, sizeof(int64_t) * (size_t) A->nvals);
+    q += sizeof(int64_t) * (size_t) A->nvals;
+    memcpy(q, hx, A->type->size * (size_t) (A->iso ? 1 : A->nvals));
+    free(hp);
+    free(hj);
+    free(hx);
+    *blob_size_handle = need;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_serialize(void **blob_handle, GrB_Index *blob_size_handle, GrB_Matrix A,
+                              const GrB_Descriptor desc)
+{
+    (void) desc;        // compression method / threads: this format is uncompressed
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!blob_handle || !blob_size_handle) return GrB_NULL_POINTER;
+    GrB_Index size = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    void *blob = malloc((size_t) size);
+    if (!blob) return GrB_OUT_OF_MEMORY;
+    info = GrB_Matrix_serialize(blob, &size, A);
+    if (info != GrB_SUCCESS)
+    {
+        free(blob);
+        return info;
+    }
+    *blob_handle = blob;
+    *blob_size_handle = size;
+    return GrB_SUCCESS;
+}
+
+// header and structure checks of a blob; no device access
+GrB_Info GB_blob_check(const void *blob, GrB_Index blob_size, GrB_Type *type_out)
+{
+    if (!blob) return GrB_NULL_POINTER;
+    if (blob_size < sizeof(uint64_t) * GB_BLOB_HDR) return GrB_INVALID_OBJECT;
+    const uint64_t *h = (const uint64_t *) blob;
+    if (h[0] != GB_BLOB_MAGIC) return GrB_INVALID_OBJECT;
+    GrB_Type t = GB_type_from_code((int32_t) h[1]);
+    if (!t || h[6] != t->size || h[2] > 1) return GrB_INVALID_OBJECT;
+    const uint64_t nrows = h[3], nvals = h[5];
+    if (nrows >= ((uint64_t) 1 << 56) || nvals >= ((uint64_t) 1 << 56)) return GrB_INVALID_OBJECT;
+    if (h[7] != GB_blob_size((int64_t) nrows, (int64_t) nvals, t->size, h[2] != 0) ||
+        h[7] > blob_size)
+        return GrB_INVALID_OBJECT;
+    // row pointers monotone from 0 to nvals, columns in range and sorted per row
+    const int64_t *p = (const int64_t *) (h + GB_BLOB_HDR);
+    const int64_t *j = p + nrows + 1;
+    if (p[0] != 0 || (uint64_t) p[nrows] != nvals) return GrB_INVALID_OBJECT;
+    for (uint64_t i = 0; i < nrows; i++)
+    {
+        if (p[i + 1] < p[i]) return GrB_INVALID_OBJECT;
+        for (int64_t k = p[i]; k < p[i + 1]; k++)
+        {
+            if (j[k] < 0 || (uint64_t) j[k] >= h[4]) return GrB_INVALID_OBJECT;
+            if (k > p[i] && j[k] <= j[k - 1]) return GrB_INVALID_OBJECT;
+        }
+    }
+    if (type_out) *type_out = t;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GrB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob, GrB_Index blob_size)
+{
+    if (!C) return GrB_NULL_POINTER;
+    *C = NULL;
+    GrB_Type bt;
+    GrB_Info info = GB_blob_check(blob, blob_size, &bt);
+    if (info != GrB_SUCCESS) return info;
+    if (type && type != bt) return GrB_DOMAIN_MISMATCH;
+    const uint64_t *h = (const uint64_t *) blob;
+    const int64_t nrows = (int64_t) h[3], ncols = (int64_t) h[4], nvals = (int64_t) h[5];
+    GrB_Matrix A;
+    if ((info = GrB_Matrix_new(&A, bt, (GrB_Index) nrows, (GrB_Index) ncols)) != GrB_SUCCESS)
+        return info;
+    const int64_t *p = (const int64_t *) (h + GB_BLOB_HDR);
+    const int64_t *j = p + nrows + 1;
+    const void *x = (const void *) (j + nvals);
+    info = GB_upload(A, p, j, x, nvals, h[2] != 0);
+    if (info != GrB_SUCCESS)
+    {
+        GrB_Matrix_free(&A);
+        return info;
+    }
+    *C = A;
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob,
+                                GrB_Index blob_size, const GrB_Descriptor desc)
+{
+    (void) desc;
+    return GrB_Matrix_deserialize(C, type, blob, blob_size);
 }

@@ -212,6 +212,18 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
                                GrB_Index *Ap_size, GrB_Index *Aj_size, GrB_Index *Ax_size,
                                bool *iso, bool *jumbled, const GrB_Descriptor desc);
 
+// Serialize to / from a blob (Include/GraphBLAS.h:10917-10990 of the
+// reference).  The blob is this library's own uncompressed CSR format; a blob
+// in the reference's format is rejected with GrB_INVALID_OBJECT.
+GrB_Info GxB_Matrix_serialize(void **blob_handle, GrB_Index *blob_size_handle, GrB_Matrix A,
+                              const GrB_Descriptor desc);
+GrB_Info GrB_Matrix_serialize(void *blob, GrB_Index *blob_size_handle, GrB_Matrix A);
+GrB_Info GrB_Matrix_serializeSize(GrB_Index *blob_size_handle, GrB_Matrix A);
+GrB_Info GxB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob,
+                                GrB_Index blob_size, const GrB_Descriptor desc);
+GrB_Info GrB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob,
+                                GrB_Index blob_size);
+
 //------------------------------------------------------------------------------
 // sparse vectors (Include/GraphBLAS.h:3623-3670, 3745, 4197 of the reference)
 //------------------------------------------------------------------------------

@@ -93,3 +93,30 @@ def test_vector_empty_frontier(grb):
     w = capi.Vector("int64", G.nrows)
     assert capi.vxm(w, None, "GxB_ANY_SECONDI_INT64", q, Gd) == 0, capi.last_error()
     assert w.nvals == 0
+
+
+@pytest.mark.parametrize("tname,iso", [("int64", False), ("fp64", False), ("bool", True),
+                                       ("int32", False)])
+def test_serialize_roundtrip(grb, tname, iso):
+    L = capi.lib()
+    rng = np.random.default_rng(4)
+    A = rand_csr(50, 70, 0.1, rng)
+    if tname == "bool":
+        A.x = None
+    else:
+        A.x = {"int64": lambda: rng.integers(-9, 9, A.nnz), "fp64": lambda: rng.random(A.nnz),
+               "int32": lambda: rng.integers(-9, 9, A.nnz).astype(np.int32)}[tname]()
+    M = capi.Matrix.from_csr(A, tname)
+    blob, size = ctypes.c_void_p(), ctypes.c_uint64()
+    assert L.GxB_Matrix_serialize(ctypes.byref(blob), ctypes.byref(size), M.h, None) == 0
+    bound = ctypes.c_uint64()
+    assert L.GrB_Matrix_serializeSize(ctypes.byref(bound), M.h) == 0 and bound.value >= size.value
+    out = ctypes.c_void_p()
+    assert L.GxB_Matrix_deserialize(ctypes.byref(out), None, blob, size.value, None) == 0
+    capi._libc.free(blob)
+    B = capi.Matrix.__new__(capi.Matrix)
+    B.h, B.type_name, B.nrows, B.ncols = out, tname, 50, 70
+    got = B.to_csr()
+    ref = M.to_csr()
+    assert np.array_equal(got.p, ref.p) and np.array_equal(got.j, ref.j)
+    assert np.array_equal(np.asarray(got.x), np.asarray(ref.x))

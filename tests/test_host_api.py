@@ -236,3 +236,33 @@ def test_vector_objects_without_gpu():
     w = capi.Vector("int64", 7)
     # u' (1 x 10) * A (10 x 10) cannot go into a vector of size 7
     assert capi.vxm(w, None, "GxB_PLUS_PAIR_INT64", v, A) == capi.GrB_DIMENSION_MISMATCH
+
+
+def test_deserialize_rejects_bad_blobs():
+    """Header and structure checks of GrB_Matrix_deserialize run on the host."""
+    capi.init()
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    hdr = np.zeros(8, dtype=np.uint64)
+    # wrong magic
+    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, hdr.ctypes.data, hdr.nbytes) == \
+        capi.GrB_INVALID_OBJECT
+    # right magic, 2x3 int64 matrix with 1 entry, but a column out of range
+    magic = int.from_bytes(b"GMCSR001", "little")
+    p = np.array([0, 1, 1], dtype=np.int64)
+    j = np.array([5], dtype=np.int64)
+    x = np.array([7], dtype=np.int64)
+    size = 8 * 8 + p.nbytes + j.nbytes + x.nbytes
+    blob = np.concatenate([np.array([magic, 8, 0, 2, 3, 1, 8, size], dtype=np.uint64).view(np.uint8),
+                           p.view(np.uint8), j.view(np.uint8), x.view(np.uint8)])
+    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, blob.ctypes.data, blob.nbytes) == \
+        capi.GrB_INVALID_OBJECT
+    # truncated blob
+    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, blob.ctypes.data, blob.nbytes - 8) == \
+        capi.GrB_INVALID_OBJECT
+    # a type that differs from the blob's
+    j[0] = 2
+    blob = np.concatenate([np.array([magic, 8, 0, 2, 3, 1, 8, size], dtype=np.uint64).view(np.uint8),
+                           p.view(np.uint8), j.view(np.uint8), x.view(np.uint8)])
+    assert L.GrB_Matrix_deserialize(ctypes.byref(out), capi.handle("GrB_FP64"), blob.ctypes.data,
+                                    blob.nbytes) == capi.GrB_DOMAIN_MISMATCH
