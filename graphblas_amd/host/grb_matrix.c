@@ -571,7 +571,7 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
 // A blob from the reference's own format is rejected with
 // GrB_INVALID_OBJECT (its header does not carry this magic).
 
-#define GB_BLOB_MAGIC 0x313030525343474dULL     // "GMCSR001"
+#define GB_BLOB_MAGIC 0x3130305253434d47ULL     // "GMCSR001" read as a little-endian uint64
 #define GB_BLOB_HDR 8
 
 static GrB_Index GB_blob_size(int64_t nrows, int64_t nvals, size_t xsize, bool iso)
