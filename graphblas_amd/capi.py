@@ -91,7 +91,7 @@ class GmSemiring(ctypes.Structure):
 class GmDot3Params(ctypes.Structure):
     _fields_ = [("thread_max", ctypes.c_int64), ("team16_max", ctypes.c_int64),
                 ("wave_max", ctypes.c_int64), ("bitmap_min_len", ctypes.c_int64),
-                ("order", ctypes.c_int64)]
+                ("order", ctypes.c_int64), ("row_min", ctypes.c_int64)]
 
 
 class GmSaxpy3Params(ctypes.Structure):

@@ -98,6 +98,7 @@ struct Dot3Args
     RowBits bA, bB;               // bitmaps of the long rows of A and B
     int32_t cols32;               // column indices fit 32 bits (LDS staging allowed)
     uint64_t *bkeys;              // NULL, or 3 x mnz grouping keys of the binned entries
+    const uint8_t *rowmode;       // NULL, or 1 per mask row handled whole by k_dot3_rows
     int64_t key_nA;               // probe-2 keys are key_nA + j
 };
 
@@ -592,7 +593,8 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                 const int64_t j = g.Mj[p];
                 bool e = false;
                 Z z = S::identity();
-                if (g.mask_struct || mask_true(g.Mx, p, g.msize, g.m_iso))
+                if (g.rowmode && g.rowmode[i]) bin = 3;          // the row kernel's entry
+                else if (g.mask_struct || mask_true(g.Mx, p, g.msize, g.m_iso))
                 {
                     Pair P;
                     const int64_t w = d3_pair(g, i, j, P);
@@ -604,7 +606,8 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                         key = (P.probe == 2) ? (uint64_t) (g.key_nA + j) : (uint64_t) i;
                     }
                 }
-                if (bin < 0)
+                if (bin == 3) bin = -2;
+                else if (bin < 0)
                 {
                     g.live[p] = e ? 1 : 0;
                     if (e && !S::iso_out) ((Z *) g.W)[p] = z;
@@ -762,6 +765,133 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint
             if (q >= hi) break;
             block_task<S>(g, list[q], s_z, s_e);
         }
+    }
+}
+
+//------------------------------------------------------------------------------
+// row mode: whole mask rows against an LDS bitmap of A(i,:)
+//------------------------------------------------------------------------------
+
+// A mask row i with enough entries, whose A(i,:) spans fewer than RM_BITS
+// columns, is done by one workgroup: A(i,:) becomes a bitmap over its column
+// range in LDS (plus a rank per 32-bit word for valued semirings), and every
+// entry (i,j) streams B(j,:), trimmed to that range, through it with
+// coalesced loads -- LDS lookups instead of random probes into a global
+// index.  An entry whose B(j,:) is long and indexed while A(i,:) is short
+// probes A(i,:) into B(j,:)'s index instead, as the binned kernels do.
+constexpr int RM_BITS = 1 << 17;              // bitmap span: 16 KB of LDS
+constexpr int RM_WORDS = RM_BITS / 32;
+constexpr int RM_TEAM = 16;                   // lanes per mask entry
+
+__global__ __launch_bounds__(D3_NT) void k_d3_rowmode(const int64_t *__restrict__ Mp,
+    const int64_t *__restrict__ Ap, const int64_t *__restrict__ Aj, int64_t n, int64_t min_mrow,
+    int64_t min_arow, uint8_t *__restrict__ flag, uint32_t *__restrict__ rows,
+    unsigned long long *__restrict__ cnt)
+{
+    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
+    const int64_t n_pad = ((n + 63) / 64) * 64;          // whole wavefronts for the appends
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n_pad; i += stride)
+    {
+        bool ok = false;
+        if (i < n)
+        {
+            const int64_t a0 = Ap[i], a1 = Ap[i + 1];
+            ok = (Mp[i + 1] - Mp[i]) >= min_mrow && (a1 - a0) >= min_arow && a1 > a0 &&
+                 (Aj[a1 - 1] - Aj[a0]) < RM_BITS;
+            flag[i] = ok ? 1 : 0;
+        }
+        const unsigned long long slot = wave_append(ok, cnt);
+        if (ok) rows[slot] = (uint32_t) i;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(D3_NT) void k_dot3_rows(Dot3Args g, const uint32_t *rows,
+                                                     const unsigned long long *nrows_ptr)
+{
+    using Z = typename S::Z;
+    __shared__ uint32_t s_bm[RM_WORDS];
+    __shared__ uint32_t s_rank[S::pattern ? 1 : RM_WORDS];
+    __shared__ int64_t s_scan[D3_NT / 64 + 1];
+    const int64_t nrows = (int64_t) *nrows_ptr;
+    const int team = threadIdx.x / RM_TEAM, lane = threadIdx.x & (RM_TEAM - 1);
+    constexpr int NTEAM = D3_NT / RM_TEAM;
+    const Z *Ax = (const Z *) g.Ax, *Bx = (const Z *) g.Bx;
+    for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x)
+    {
+        const int64_t i = rows[r];
+        const int64_t a0 = g.Ap[i], a1 = g.Ap[i + 1];
+        const int64_t alo = g.Aj[a0], ahi = g.Aj[a1 - 1];
+        const int nw = (int) ((ahi - alo) >> 5) + 1;
+        for (int w = threadIdx.x; w < nw; w += D3_NT) s_bm[w] = 0;
+        __syncthreads();
+        for (int64_t q = a0 + threadIdx.x; q < a1; q += D3_NT)
+        {
+            const int64_t d = g.Aj[q] - alo;
+            atomicOr(&s_bm[d >> 5], 1u << (d & 31));
+        }
+        __syncthreads();
+        if constexpr (!S::pattern)
+        {
+            // rank[w] = entries of A(i,:) in words before w
+            const int chunk = (nw + D3_NT - 1) / D3_NT;
+            const int w0 = threadIdx.x * chunk, w1 = (w0 + chunk < nw) ? w0 + chunk : nw;
+            int64_t c = 0;
+            for (int w = w0; w < w1; w++) c += __popc(s_bm[w]);
+            int64_t tot;
+            int64_t pre = block_exclusive_scan<int64_t, D3_NT>(c, s_scan, &tot);
+            for (int w = w0; w < w1; w++)
+            {
+                s_rank[w] = (uint32_t) pre;
+                pre += __popc(s_bm[w]);
+            }
+            __syncthreads();
+        }
+        const int64_t m0 = g.Mp[i], m1 = g.Mp[i + 1];
+        // every team runs the same number of rounds (team-wide shuffles inside)
+        for (int64_t pb0 = m0; pb0 < m1; pb0 += NTEAM)
+        {
+            const int64_t p = pb0 + team;
+            bool e = false;
+            Z z = S::identity();
+            if (p < m1)
+            {
+                const int64_t j = g.Mj[p];
+                if (g.mask_struct || mask_true(g.Mx, p, g.msize, g.m_iso))
+                {
+                    Pair P;
+                    (void) d3_pair(g, i, j, P);
+                    if (!P.empty && P.probe == 2)
+                    {
+                        dot_lane<S, RM_TEAM>(g, P, lane, e, z);
+                    }
+                    else if (!P.empty)
+                    {
+                        const int64_t q0 = team_lower_bound<RM_TEAM>(g.Bj, P.pb, P.pb_end, alo, lane);
+                        const int64_t q1 = team_lower_bound<RM_TEAM>(g.Bj, q0, P.pb_end, ahi + 1, lane);
+                        for (int64_t q = q0 + lane; q < q1; q += RM_TEAM)
+                        {
+                            const int64_t x = g.Bj[q];
+                            const int64_t d = x - alo;
+                            const uint32_t word = s_bm[d >> 5];
+                            const uint32_t bit = 1u << (d & 31);
+                            if (!(word & bit)) continue;
+                            int64_t qa = a0;
+                            if constexpr (!S::pattern) qa = a0 + (int64_t) s_rank[d >> 5] + __popc(word & (bit - 1u));
+                            Z t = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, q, g.b_iso), x);
+                            if (!e) { z = t; e = true; }
+                            else if constexpr (S::any_min_k) { if (t < z) z = t; }
+                            else z = S::upd(z, t);
+                            if constexpr (S::is_any && !S::any_min_k) break;
+                            if (S::terminal(z) && !S::any_min_k) break;
+                        }
+                    }
+                }
+            }
+            team_reduce<S, RM_TEAM>(e, z);
+            if (lane == 0 && p < m1) d3_store<S>(g, p, e, z);
+        }
+        __syncthreads();
     }
 }
 
@@ -1213,6 +1343,24 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
             GM_TRY(dot3_bitmaps(g, A, B, !S::pattern, minlen, bm_offA, bm_offB, bm_words, bm_rank,
                                 bm_list, bm_ctr));
         k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
+        // whole rows to the row kernel: mask rows with at least row_min
+        // entries whose A row spans fewer than RM_BITS columns
+        const int64_t row_min = (prm && prm->row_min != 0) ? prm->row_min
+                              : (tu.row_min != 0 ? tu.row_min : -1);
+        g.rowmode = nullptr;
+        DevBuf rm_flag, rm_rows, rm_cnt;
+        if (row_min > 0 && g.cols32)
+        {
+            if (!rm_flag.alloc(n) || !rm_rows.alloc(sizeof(uint32_t) * n) || !rm_cnt.alloc(8))
+                return set_error(GM_OUT_OF_MEMORY, "dot3: row mode");
+            k_zero_u64<<<1, 64, 0, st>>>(rm_cnt.as<unsigned long long>(), 1);
+            ProfScope ps("dot3_rowmode");
+            k_d3_rowmode<<<grid_for(n, D3_NT, cap), D3_NT, 0, st>>>(
+                M->p, A->p, A->j, n, row_min, 8, rm_flag.as<uint8_t>(), rm_rows.as<uint32_t>(),
+                rm_cnt.as<unsigned long long>());
+            GM_HIP(hipGetLastError(), "k_d3_rowmode");
+            g.rowmode = rm_flag.as<uint8_t>();
+        }
         {
             ProfScope ps("dot3_phase1");
             k_dot3_phase1<S><<<grid_for(Mnz, D3_P1_TILE, cap), D3_NT, 0, st>>>(g);
@@ -1240,6 +1388,13 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
             ProfScope ps("dot3_block");
             k_dot3_block<S><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(g);
             GM_HIP(hipGetLastError(), "k_dot3_block");
+        }
+        if (g.rowmode)
+        {
+            ProfScope ps("dot3_rows");
+            k_dot3_rows<S><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                g, rm_rows.as<uint32_t>(), rm_cnt.as<unsigned long long>());
+            GM_HIP(hipGetLastError(), "k_dot3_rows");
         }
         // compaction
         const int64_t ntiles = (Mnz + D3_CT_TILE - 1) / D3_CT_TILE;

@@ -290,7 +290,7 @@ using namespace gm;
 
 namespace gm
 {
-static gm_dot3_params g_d3_tuned = {0, 0, 0, 0, 0};
+static gm_dot3_params g_d3_tuned = {0, 0, 0, 0, 0, 0};
 static gm_saxpy3_params g_sx_tuned = {0, 0, 0, 0};
 
 gm_dot3_params dot3_tuned()
@@ -311,7 +311,7 @@ extern "C" {
 int gm_dot3_set_defaults(const gm_dot3_params *p)
 {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_d3_tuned = p ? *p : gm_dot3_params{0, 0, 0, 0, 0};
+    g_d3_tuned = p ? *p : gm_dot3_params{0, 0, 0, 0, 0, 0};
     return GM_SUCCESS;
 }
 
@@ -347,6 +347,7 @@ int gm_load_tuning(const char *path)
         else if (k == "dot3.wave_max") d.wave_max = v;
         else if (k == "dot3.bitmap_min_len") d.bitmap_min_len = v;
         else if (k == "dot3.order") d.order = v;
+        else if (k == "dot3.row_min") d.row_min = v;
         else if (k == "saxpy3.wave_small_max") x.wave_small_max = v;
         else if (k == "saxpy3.wave_large_max") x.wave_large_max = v;
         else if (k == "saxpy3.block_max") x.block_max = v;

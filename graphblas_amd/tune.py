@@ -30,9 +30,10 @@ DOT3_SPACE = {
     "wave_max": [2048, 4096, 8192],
     "bitmap_min_len": [64, 128, 256, 512, -1],
     "order": [1, 2],
+    "row_min": [16, 64, 256, 1024],
 }
 DOT3_BUILTIN = {"thread_max": 64, "team16_max": 512, "wave_max": 4096, "bitmap_min_len": 128,
-                "order": 0}
+                "order": 0, "row_min": -1}
 SAXPY3_SPACE = {
     "wave_small_max": [16, 32, 64],
     "wave_large_max": [128, 256, 512],
@@ -203,7 +204,7 @@ def main(argv=None):
 
         def make(d):
             return capi.GmDot3Params(d["thread_max"], d["team16_max"], d["wave_max"],
-                                     d["bitmap_min_len"], d["order"])
+                                     d["bitmap_min_len"], d["order"], d["row_min"])
     try:
         cur, t0, t1 = tune(wl, space, builtin, make, reps=args.reps,
                            log=lambda m: print(m, flush=True))

@@ -113,6 +113,10 @@ typedef struct
     int64_t order;        // order of the team16 / wave entries: 0 auto (grouped
                           // when nnz(M) >= 2^20), 1 mask order, 2 grouped by
                           // the row whose probe index they read
+    int64_t row_min;      // mask rows with at least row_min entries whose A row
+                          // spans < 2^17 columns are done whole against an LDS
+                          // bitmap of that row (-1 = never by default: measured
+                          // slower than the bins on the batch and RMAT 14/22)
 } gm_dot3_params;
 
 // saxpy3 tuning knobs (0 = default).  f = flops of a row; t = its table
@@ -169,7 +173,7 @@ int gm_synchronize(void);
 // gm_saxpy3_params left 0 (and for calls that pass NULL, as GrB_mxm does);
 // NULL restores the built-in values.  gm_load_tuning reads "key value" lines
 // (dot3.thread_max, dot3.team16_max, dot3.wave_max, dot3.bitmap_min_len,
-// dot3.order, saxpy3.wave_small_max, saxpy3.wave_large_max,
+// dot3.order, dot3.row_min, saxpy3.wave_small_max, saxpy3.wave_large_max,
 // saxpy3.block_max, saxpy3.mode; '#' starts a comment line) as written by
 // the autotuner (graphblas_amd/tune.py); gm_init loads the file named by the
 // environment variable GM_TUNING_FILE.  No counterpart in the reference.

@@ -78,7 +78,8 @@ def test_dot3_forced_bins(grb, params, handle):
     M = power_law_csr(500, 15, rng, values="int64")
     name, _ = SR[handle]
     ref = O.dot3(name, M, A, B)
-    got = gm_dot3(handle, M, A, B, params=params)
+    # row mode off, so the chosen bins see every entry
+    got = gm_dot3(handle, M, A, B, params=tuple(params) + (0, -1))
     assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=str(params))
 
 
@@ -158,7 +159,7 @@ def test_dot3_entry_order(grb, order, params, handle):
     M = power_law_csr(700, 20, rng, values="int64")
     name, _ = SR[handle]
     ref = O.dot3(name, M, A, B)
-    got = gm_dot3(handle, M, A, B, params=tuple(params) + (order,))
+    got = gm_dot3(handle, M, A, B, params=tuple(params) + (order, -1))
     assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None, what=f"order {order}")
 
 
@@ -170,3 +171,35 @@ def test_dot3_grouped_tricount_default(grb):
     ref = O.dot3("plus_pair_int64", L, L, U)
     got = gm_dot3("GxB_PLUS_PAIR_INT64", L, L, U, abtype=1, mtype=1)
     assert_csr_equal(got, ref, what="tricount scale 17")
+
+
+@pytest.mark.parametrize("row_min", [1, 4])
+@pytest.mark.parametrize("bitmaps", [-1, 8])
+@pytest.mark.parametrize("handle", list(SR))
+def test_dot3_row_mode(grb, row_min, bitmaps, handle):
+    """Whole mask rows against an LDS bitmap of A(i,:) (with the probe of
+    short A rows into long indexed B rows mixed in) give the oracle's result."""
+    print(f"[dot3] row mode {row_min} {bitmaps} {handle}", flush=True)
+    rng = np.random.default_rng(hash((row_min, bitmaps, handle)) % 2 ** 32)
+    vt = _vals(handle)
+    A = power_law_csr(600, 30, rng, values=vt)
+    B = power_law_csr(600, 30, rng, values=vt)
+    M = power_law_csr(600, 25, rng, values="int64")
+    name, _ = SR[handle]
+    for struct in (True, False):
+        ref = O.dot3(name, M, A, B, mask_struct=struct)
+        got = gm_dot3(handle, M, A, B, mask_struct=struct, params=(0, 0, 0, bitmaps, 0, row_min))
+        assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None,
+                         what=f"row mode {row_min} struct={struct}")
+
+
+def test_dot3_row_mode_wide_rows(grb):
+    """A rows spanning more than 2^17 columns stay with the binned kernels."""
+    rng = np.random.default_rng(8)
+    n = 300000
+    A = rand_csr(200, n, 0.0005, rng, "int32")
+    B = rand_csr(150, n, 0.0005, rng, "int32")
+    M = rand_csr(200, 150, 0.3, rng, "int64")
+    ref = O.dot3("min_plus_int32", M, A, B)
+    got = gm_dot3("GrB_MIN_PLUS_SEMIRING_INT32", M, A, B, params=(0, 0, 0, 0, 0, 1))
+    assert_csr_equal(got, ref, what="wide rows")
