@@ -554,6 +554,8 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
 {
     using Z = typename S::Z;
     __shared__ int64_t s_rows[2];
+    // the tile's row pointers Mp[r0 .. r1+1], when the window is short
+    __shared__ int64_t s_mp[D3_P1_TILE + 2];
     const int64_t ntiles = (g.mnz + D3_P1_TILE - 1) / D3_P1_TILE;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
     {
@@ -573,6 +575,10 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
         }
         __syncthreads();
         const int64_t r0 = s_rows[0], r1 = s_rows[1];
+        const bool lds_rows = (r1 - r0 + 2) <= (int64_t) (D3_P1_TILE + 2);
+        if (lds_rows)
+            for (int64_t t = threadIdx.x; t <= r1 - r0 + 1; t += D3_NT) s_mp[t] = g.Mp[r0 + t];
+        __syncthreads();
         #pragma unroll 1
         for (int u = 0; u < D3_P1_PER_THREAD; u++)
         {
@@ -584,10 +590,21 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
             if (valid)
             {
                 int64_t lo = r0, hi = r1;
-                while (lo < hi)
+                if (lds_rows)
                 {
-                    const int64_t mid = (lo + hi + 1) >> 1;
-                    if (g.Mp[mid] <= p) lo = mid; else hi = mid - 1;
+                    while (lo < hi)
+                    {
+                        const int64_t mid = (lo + hi + 1) >> 1;
+                        if (s_mp[mid - r0] <= p) lo = mid; else hi = mid - 1;
+                    }
+                }
+                else
+                {
+                    while (lo < hi)
+                    {
+                        const int64_t mid = (lo + hi + 1) >> 1;
+                        if (g.Mp[mid] <= p) lo = mid; else hi = mid - 1;
+                    }
                 }
                 i = lo;
                 const int64_t j = g.Mj[p];
