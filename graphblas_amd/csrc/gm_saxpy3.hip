@@ -703,7 +703,7 @@ constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 22;
-constexpr int LB_WIN = 16;           // look-back window: LB_WIN x 64 predecessors
+constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecessors (one wavefront's loads per step measured fastest)
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
