@@ -707,13 +707,16 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
+// Four waves per SIMD (<= 128 VGPRs): the look-back of one workgroup is
+// hidden behind the other workgroups of the CU.
 template <class S, class K, int NT>
-__global__ __launch_bounds__(NT, 768 / NT) void k_sx_fused(SxArgs g, int64_t fs,
+__global__ __launch_bounds__(NT, 1024 / NT) void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
                                                          int *err)
 {
     using Z = typename S::Z;
     constexpr int FUSED_ROWS = NT / TG;
+    static_assert(FUSED_ROWS <= 64, "the tile's row offsets are scanned by one wavefront");
     // per-team rows padded by one element so the 8 teams of a wavefront
     // start on different LDS banks
     __shared__ K s_key[FUSED_ROWS][FMAX + 1];         // sort keys, then output columns (int32)
@@ -912,16 +915,22 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_sx_fused(SxArgs g, int64_t fs,
             // wavefront 0: tile offsets, then a look-back over up to 64
             // predecessors per step (lane l reads tile - 1 - l)
             const int lane = threadIdx.x;
-            int64_t run = 0;
-            for (int t = 0; t < FUSED_ROWS; t++) run += s_cnt[t];
-            if (lane == 0)
+            // row offsets of the tile: inclusive scan of the row counts
+            // across the wavefront's lanes
+            const int64_t c = lane < FUSED_ROWS ? s_cnt[lane] : 0;
+            int64_t incl = c;
+            #pragma unroll
+            for (int off = 1; off < FUSED_ROWS; off <<= 1)
             {
-                int64_t r2 = 0;
-                for (int t = 0; t < FUSED_ROWS; t++) { s_off[t] = r2; r2 += s_cnt[t]; }
-                s_off[FUSED_ROWS] = r2;
+                const int64_t o = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += o;
+            }
+            const int64_t run = __shfl(incl, FUSED_ROWS - 1, 64);
+            if (lane == 0)
                 __hip_atomic_store(&status[tile], (tile == 0 ? LB_PRE : LB_AGG) | (unsigned long long) run,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (lane < FUSED_ROWS) s_off[lane] = incl - c;
+            if (lane == 0) s_off[FUSED_ROWS] = run;
             int64_t prefix = 0;
             if (tile > 0)
             {
