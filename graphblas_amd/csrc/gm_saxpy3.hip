@@ -67,6 +67,9 @@ struct SxArgs
     void *gvals;
     uint64_t *glist;
     int64_t ws_max, wl_max, blk_max;
+    // B's entries as 16-byte (column, value bits) pairs, or null: one B row
+    // of the gather then spans one contiguous range instead of two
+    const int64_t *Bjx;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -181,11 +184,18 @@ __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__
     __shared__ unsigned long long s[2][SX_NT / 64];
     unsigned long long ma = 0, mb = 0;
     const int64_t lim = m > kb ? m : kb;
-    for (int64_t i = (int64_t) blockIdx.x * SX_NT + threadIdx.x; i < lim;
-         i += (int64_t) gridDim.x * SX_NT)
+    // four consecutive rows per thread and pass, so that one pass of a
+    // modest grid has all of its loads in flight together
+    for (int64_t i0 = ((int64_t) blockIdx.x * SX_NT + threadIdx.x) * 4; i0 < lim;
+         i0 += (int64_t) gridDim.x * SX_NT * 4)
     {
-        if (i < m) { unsigned long long l = (unsigned long long) (Ap[i + 1] - Ap[i]); ma = l > ma ? l : ma; }
-        if (i < kb) { unsigned long long l = (unsigned long long) (Bp[i + 1] - Bp[i]); mb = l > mb ? l : mb; }
+        #pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+            const int64_t i = i0 + u;
+            if (i < m) { unsigned long long l = (unsigned long long) (Ap[i + 1] - Ap[i]); ma = l > ma ? l : ma; }
+            if (i < kb) { unsigned long long l = (unsigned long long) (Bp[i + 1] - Bp[i]); mb = l > mb ? l : mb; }
+        }
     }
     #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -203,8 +213,12 @@ __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__
             ma = s[0][w] > ma ? s[0][w] : ma;
             mb = s[1][w] > mb ? s[1][w] : mb;
         }
-        atomicMax(&out[0], ma);
-        atomicMax(&out[1], mb);
+        // an atomic only where this block raises the maximum seen so far
+        // (a stale read only costs an atomic that was not needed)
+        const unsigned long long ca = __hip_atomic_load(&out[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long cb = __hip_atomic_load(&out[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ma > ca) atomicMax(&out[0], ma);
+        if (mb > cb) atomicMax(&out[1], mb);
     }
 }
 
@@ -697,6 +711,20 @@ __global__ void k_sx_iso_one(void *x, int size)
 // only waits on tiles already running; a bounded spin turns a lost
 // predecessor into an error flag instead of a hang.
 
+// B packed as (column, value bits) pairs for the fused kernel's gathers: for
+// 8-byte values that the semiring reads
+template <class S>
+constexpr bool SX_PACK_OK = sizeof(typename S::Z) == 8 && !S::pattern;
+
+__global__ __launch_bounds__(256) void k_sx_pack(const int64_t *__restrict__ Bj,
+                                                 const int64_t *__restrict__ Bx, int64_t nnz,
+                                                 longlong2 *__restrict__ out)
+{
+    for (int64_t e = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t) gridDim.x * blockDim.x)
+        out[e] = make_longlong2(Bj[e], Bx[e]);
+}
+
 constexpr int FUSED_NT = 256;              // threads of a fused-kernel workgroup
 constexpr int FUSED_ROWS = FUSED_NT / TG;  // rows of a tile
 constexpr unsigned long long LB_AGG = 1ull << 62;
@@ -707,10 +735,11 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
-// Four waves per SIMD (<= 128 VGPRs): the look-back of one workgroup is
-// hidden behind the other workgroups of the CU.
-template <class S, class K, int NT>
-__global__ __launch_bounds__(NT, 1024 / NT) void k_sx_fused(SxArgs g, int64_t fs,
+// Four waves per SIMD (<= 128 VGPRs) with 32-bit keys: the look-back of one
+// workgroup is hidden behind the other workgroups of the CU.  64-bit keys
+// double the key LDS, which then allows three.
+template <class S, class K, int NT, bool PK>
+__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1024 : 768) / NT) void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
                                                          int *err)
 {
@@ -785,14 +814,25 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_sx_fused(SxArgs g, int64_t fs
                             if (s_pre[team][mid] <= q) lo = mid; else hi = mid - 1;
                         }
                         const int64_t pb = s_bs[team][lo] + (q - s_pre[team][lo]);
-                        const int64_t j = g.Bj[pb];
+                        int64_t j;
+                        Z bv;
+                        if constexpr (PK)
+                        {
+                            const longlong2 jx = *(const longlong2 *) (g.Bjx + 2 * pb);
+                            j = jx.x;
+                            bv = __builtin_bit_cast(Z, jx.y);
+                        }
+                        else
+                            j = g.Bj[pb];
                         if (sx_mask_ok(g, pm0, pm1, j))
                         {
                             const int64_t fq = f + q;
                             key[fq] = (K) (((K) j << 6) | (K) fq);
                             if constexpr (!S::iso_out)
-                                val[fq] = S::mul(s_av[team][lo], S::geta((const Z *) g.Bx, pb, g.b_iso),
-                                                 s_k[team][lo]);
+                            {
+                                if constexpr (!PK) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
+                                val[fq] = S::mul(s_av[team][lo], bv, s_k[team][lo]);
+                            }
                             nallow++;
                         }
                     }
@@ -1106,6 +1146,7 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     g.rows = rows.as<uint32_t>(); g.cnt = cnt.as<unsigned long long>();
     g.gtab_off = goff.as<int64_t>(); g.gtab_log2 = glog.as<int32_t>();
     g.gkeys = nullptr; g.gvals = nullptr; g.glist = nullptr;
+    g.Bjx = nullptr;
     const gm_saxpy3_params tu = saxpy3_tuned();
     g.ws_max = pick_param(prm ? prm->wave_small_max : 0, tu.wave_small_max, 64);
     g.wl_max = pick_param(prm ? prm->wave_large_max : 0, tu.wave_large_max, 512);
@@ -1140,7 +1181,7 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     // When A has far fewer rows than B (a few frontier rows against a large
     // graph) the flop pass below is as cheap as this one, so it is skipped
     // together with its synchronisation.
-    int64_t maxA = 0, maxB = 0, nnzA = 0;
+    int64_t maxA = 0, maxB = 0, nnzA = 0, nnzB = -1;
     const bool probe_small = m * 8 >= B->nrows;
     k_sx_fill_u64<<<1, 64, 0, st>>>(g.cnt, 16);
     GM_HIP(hipGetLastError(), "k_sx_fill_u64");
@@ -1149,17 +1190,20 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         {
             ProfScope ps("sx_maxlen");
             const int64_t lim = m > B->nrows ? m : B->nrows;
-            k_sx_maxlen<<<sx_grid(lim, SX_NT, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
+            k_sx_maxlen<<<sx_grid((lim + 3) / 4, SX_NT, (int64_t) cu_count() * 8), SX_NT, 0, st>>>(
                 A->p, m, B->p, B->nrows, g.cnt + 8);
             GM_HIP(hipGetLastError(), "k_sx_maxlen");
         }
         GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
                "maxlen d2h");
         GM_HIP(hipMemcpyAsync(hp + 10, A->p + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
+        GM_HIP(hipMemcpyAsync(hp + 13, B->p + B->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+               "nnz(B)");
         GM_HIP(hipStreamSynchronize(st), "maxlen sync");
         maxA = (int64_t) hp[8];
         maxB = (int64_t) hp[9];
         nnzA = (int64_t) hp[10];
+        nnzB = (int64_t) hp[13];
     }
 
     bool fused = (mode != 2);
@@ -1239,16 +1283,43 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
             status.as<unsigned long long>(), ntiles + 1);
         k_sx_fill_u64<<<1, 64, 0, st>>>((unsigned long long *) g.cnt + 12, 1);
         GM_HIP(hipGetLastError(), "k_sx_fill_u64");
+        // B as 16-byte (column, value) pairs when its rows are gathered at
+        // least about once each on average (the pack reads and writes B once)
+        DevBuf bjx;
+        if constexpr (SX_PACK_OK<S>)
+        {
+            if (!B->iso && B->x && nnzB > 0 && 2 * nnzA >= B->nrows &&
+                bjx.alloc(sizeof(longlong2) * nnzB))
+            {
+                ProfScope ps("sx_pack");
+                k_sx_pack<<<sx_grid(nnzB, 256, (int64_t) cu_count() * 16), 256, 0, st>>>(
+                    B->j, (const int64_t *) B->x, nnzB, bjx.as<longlong2>());
+                GM_HIP(hipGetLastError(), "k_sx_pack");
+                g.Bjx = bjx.as<int64_t>();
+            }
+        }
         {
             ProfScope ps("sx_fused");
             const int64_t gcap = (int64_t) cu_count() * 8 * (256 / FUSED_NT);
             const int64_t grid = ntiles < gcap ? ntiles : gcap;
-            if (B->ncols <= ((int64_t) 1 << 26))
-                k_sx_fused<S, uint32_t, FUSED_NT><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                    g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
-            else
-                k_sx_fused<S, uint64_t, FUSED_NT><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                    g, fs, status.as<unsigned long long>(), ntiles, (int *) (g.cnt + 12));
+            unsigned long long *stp = status.as<unsigned long long>();
+            int *errp = (int *) (g.cnt + 12);
+            const bool k32 = B->ncols <= ((int64_t) 1 << 26);
+            if constexpr (SX_PACK_OK<S>)
+            {
+                if (g.Bjx && k32)
+                    k_sx_fused<S, uint32_t, FUSED_NT, true><<<(unsigned) grid, FUSED_NT, 0, st>>>(
+                        g, fs, stp, ntiles, errp);
+                else if (g.Bjx)
+                    k_sx_fused<S, uint64_t, FUSED_NT, true><<<(unsigned) grid, FUSED_NT, 0, st>>>(
+                        g, fs, stp, ntiles, errp);
+            }
+            if (!g.Bjx && k32)
+                k_sx_fused<S, uint32_t, FUSED_NT, false><<<(unsigned) grid, FUSED_NT, 0, st>>>(
+                    g, fs, stp, ntiles, errp);
+            else if (!g.Bjx)
+                k_sx_fused<S, uint64_t, FUSED_NT, false><<<(unsigned) grid, FUSED_NT, 0, st>>>(
+                    g, fs, stp, ntiles, errp);
             GM_HIP(hipGetLastError(), "k_sx_fused");
         }
         GM_HIP(hipMemcpyAsync(hp + 11, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");
