@@ -193,6 +193,7 @@ def lib():
         "GB_mxm_select": ([ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool,
                            ctypes.c_int], ctypes.c_int),
         "GB_semiring_codes": ([P, ctypes.POINTER(GmSemiring)], ctypes.c_bool),
+        "GB_mxm_prefer_dot3": ([ctypes.c_int64] * 6, ctypes.c_bool),
         # kernel layer
         "gm_init": ([], ctypes.c_int),
         "gm_finalize": ([], None),
@@ -256,6 +257,8 @@ def lib():
         "GrB_vxm": ([P, P, P, P, P, P, P], ctypes.c_int),
         "GrB_mxv": ([P, P, P, P, P, P, P], ctypes.c_int),
         "gm_transpose": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr)], ctypes.c_int),
+        "gm_flopcount": ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr)],
+                         ctypes.c_int),
         "gm_accum_mask": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                            ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr),
                            I32], ctypes.c_int),
@@ -284,9 +287,39 @@ def handle(name: str) -> ctypes.c_void_p:
     return ctypes.c_void_p.in_dll(lib(), name).value
 
 
-def desc(name: Optional[str]):
+class Descriptor:
+    """A GrB_Descriptor built field by field, e.g. Descriptor(mask_struct=True,
+    axb=GxB_AxB_SAXPY); free() releases it."""
+
+    def __init__(self, mask_struct=False, mask_comp=False, replace=False, axb=0):
+        self.h = ctypes.c_void_p()
+        _check(lib().GrB_Descriptor_new(ctypes.byref(self.h)), "GrB_Descriptor_new")
+        fields = []
+        if replace:
+            fields.append((0, 1))                     # GrB_OUTP = GrB_REPLACE
+        if mask_struct or mask_comp:
+            fields.append((1, (4 if mask_struct else 0) + (2 if mask_comp else 0)))
+        if axb:
+            fields.append((7090, axb))                # GxB_AxB_METHOD
+        for f, v in fields:
+            _check(lib().GrB_Descriptor_set(self.h, f, v), "GrB_Descriptor_set")
+
+    def free(self):
+        if self.h:
+            lib().GrB_Descriptor_free(ctypes.byref(self.h))
+            self.h = ctypes.c_void_p()
+
+
+GxB_AxB_GUSTAVSON, GxB_AxB_DOT, GxB_AxB_HASH, GxB_AxB_SAXPY = 7081, 7083, 7084, 7085
+
+
+def desc(name):
+    """A descriptor argument: None, a Descriptor, or the suffix of a
+    predefined GrB_DESC_* ("S", "RSC", "T1", ...)."""
     if name is None:
         return None
+    if isinstance(name, Descriptor):
+        return name.h
     return handle("GrB_DESC_" + name)
 
 
@@ -475,7 +508,7 @@ def mxv(w: Vector, mask: Optional[Vector], semiring: str, A: Matrix, u: Vector,
 
 
 def mxm(C: Matrix, M: Optional[Matrix], semiring: str, A: Matrix, B: Matrix,
-        descriptor: Optional[str] = None, accum=None) -> int:
+        descriptor=None, accum=None) -> int:
     """GrB_mxm(C, M, accum, semiring, A, B, desc); returns the GrB_Info code."""
     return lib().GrB_mxm(C.h, M.h if M is not None else None, accum, handle(semiring),
                          A.h, B.h, desc(descriptor))

@@ -1366,6 +1366,87 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
 
 using namespace gm;
 
+//------------------------------------------------------------------------------
+// total multiplies of C=A*B: the sum over A's entries (i,k) of |B(k,:)|.
+// Partial sums per workgroup, then one workgroup adds them (no atomics).
+//------------------------------------------------------------------------------
+
+constexpr int FLOPS_NT = 256;
+constexpr int FLOPS_MAXG = 2048;
+
+__global__ __launch_bounds__(FLOPS_NT) void k_flopcount(const int64_t *__restrict__ Aj, int64_t nnz,
+                                                        const int64_t *__restrict__ Bp,
+                                                        unsigned long long *part)
+{
+    __shared__ unsigned long long s[FLOPS_NT / 64];
+    unsigned long long f = 0;
+    for (int64_t e = (int64_t) blockIdx.x * FLOPS_NT + threadIdx.x; e < nnz;
+         e += (int64_t) gridDim.x * FLOPS_NT)
+    {
+        const int64_t k = Aj[e];
+        f += (unsigned long long) (Bp[k + 1] - Bp[k]);
+    }
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = f;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < FLOPS_NT / 64; w++) f += s[w];
+        part[blockIdx.x] = f;
+    }
+}
+
+__global__ __launch_bounds__(FLOPS_NT) void k_flopcount_sum(unsigned long long *part, int n)
+{
+    __shared__ unsigned long long s[FLOPS_NT / 64];
+    unsigned long long f = 0;
+    for (int t = threadIdx.x; t < n; t += FLOPS_NT) f += part[t];
+    #pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f += __shfl_xor(f, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = f;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int w = 1; w < FLOPS_NT / 64; w++) f += s[w];
+        part[FLOPS_MAXG] = f;
+    }
+}
+
+extern "C" int gm_flopcount(int64_t *flops, const gm_csr *A, const gm_csr *B)
+{
+    if (!flops || !A || !B) return set_error(GM_INVALID_VALUE, "gm_flopcount: NULL argument");
+    if (A->ncols != B->nrows)
+        return set_error(GM_INVALID_VALUE, "gm_flopcount: A is %lld x %lld, B is %lld x %lld",
+                         (long long) A->nrows, (long long) A->ncols, (long long) B->nrows,
+                         (long long) B->ncols);
+    *flops = 0;
+    if (A->nrows == 0 || B->nrows == 0) return GM_SUCCESS;
+    if (!A->p || !B->p) return set_error(GM_INVALID_VALUE, "gm_flopcount: NULL row pointers");
+    GM_TRY(ensure_init());
+    hipStream_t st = stream();
+    unsigned long long *hp = (unsigned long long *) pinned_scratch(16 * sizeof(unsigned long long));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(hp, A->p + A->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
+    GM_HIP(hipStreamSynchronize(st), "nnz(A) sync");
+    const int64_t nnz = (int64_t) hp[0];
+    if (nnz <= 0) return GM_SUCCESS;
+    if (!A->j) return set_error(GM_INVALID_VALUE, "gm_flopcount: NULL column indices");
+    DevBuf part;
+    if (!part.alloc(sizeof(unsigned long long) * (FLOPS_MAXG + 1)))
+        return set_error(GM_OUT_OF_MEMORY, "gm_flopcount: workspace");
+    const unsigned grid = sx_grid(nnz, FLOPS_NT * 4, FLOPS_MAXG);
+    k_flopcount<<<grid, FLOPS_NT, 0, st>>>(A->j, nnz, B->p, part.as<unsigned long long>());
+    GM_HIP(hipGetLastError(), "k_flopcount");
+    k_flopcount_sum<<<1, FLOPS_NT, 0, st>>>(part.as<unsigned long long>(), (int) grid);
+    GM_HIP(hipGetLastError(), "k_flopcount_sum");
+    GM_HIP(hipMemcpyAsync(hp, part.as<unsigned long long>() + FLOPS_MAXG, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st), "flops d2h");
+    GM_HIP(hipStreamSynchronize(st), "flops sync");
+    *flops = (int64_t) hp[0];
+    return GM_SUCCESS;
+}
+
 extern "C" bool gm_saxpy3_branch(const gm_csr *M, bool mask_comp, bool mask_struct,
                                  const gm_csr *A, const gm_csr *B, const gm_semiring *sr)
 {

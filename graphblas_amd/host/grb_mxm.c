@@ -31,6 +31,23 @@ GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran,
     return GB_METHOD_SAXPY3;
 }
 
+// C<M>=A*B with a plain mask and no method hint: the reference runs masked
+// saxpy3 here (GB_AxB_meta.c:562), whose work is the flop count of A*B.  On
+// this device the masked dot product on B' gives the same entries with work
+// about nnz(M) * (mean |A(i,:)| + mean |B(:,j)|) plus a sort of B for the
+// transpose, so dot3 is taken when that is well below the flop count (the
+// triangle count C<L>=L*L, for example).  Each C(i,j) sums the same terms in
+// the same k order either way.
+bool GB_mxm_prefer_dot3(int64_t flops, int64_t nnz_m, int64_t nnz_a, int64_t m, int64_t nnz_b,
+                        int64_t n)
+{
+    if (nnz_m <= 0 || flops <= 0) return false;
+    const double mean_a = m > 0 ? (double) nnz_a / (double) m : 0.0;
+    const double mean_b = n > 0 ? (double) nnz_b / (double) n : 0.0;
+    const double dot_work = (double) nnz_m * (mean_a + mean_b) + 16.0 * (double) nnz_b;
+    return (double) flops > 4.0 * dot_work;
+}
+
 // the gm_csr view of a device result (no copy)
 static gm_csr GB_view_of(const gm_csr_out *R)
 {
@@ -132,7 +149,7 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
     // T reaches C through GB_accum_mask when C's old entries matter
     const bool post = C->nvals > 0 && (accum != NULL || (Mask != NULL && !C_replace));
 
-    const GB_method method = GB_mxm_select(Mask != NULL, mask_comp, a_tran, b_tran, axb);
+    GB_method method = GB_mxm_select(Mask != NULL, mask_comp, a_tran, b_tran, axb);
 
     gm_csr va, vb, vm, vc;
     if ((info = GB_matrix_view(&va, A)) != GrB_SUCCESS) return GB_error(C, info, "%s", A->err);
@@ -155,6 +172,29 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         rc = gm_transpose(&AT, &va);
         if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
         va = GB_view_of(&AT);
+    }
+    if (method == GB_METHOD_SAXPY3 && Mask && !mask_comp && !b_tran && axb == GxB_DEFAULT)
+    {
+        gm_csr bt_shape = vb;       // B' by its dimensions only, for the dot3 case check
+        bt_shape.nrows = vb.ncols;
+        bt_shape.ncols = vb.nrows;
+        if (gm_dot3_branch(&vm, mstruct, &va, &bt_shape, &sr))
+        {
+            int64_t flops = 0;
+            rc = gm_flopcount(&flops, &va, &vb);
+            if (rc != GM_SUCCESS)
+            {
+                GB_free_out(&AT);
+                return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+            }
+            const int64_t nnz_a = a_tran ? AT.nvals : A->nvals;
+            if (GB_mxm_prefer_dot3(flops, Mask->nvals, nnz_a, va.nrows, B->nvals, vb.ncols))
+            {
+                if (GB_burble)
+                    printf("burble: GrB_mxm: C<M>=A*B with %lld flops: dot3 on B'\n", (long long) flops);
+                method = GB_METHOD_DOT3;
+            }
+        }
     }
     if (b_tran != (method == GB_METHOD_DOT3))
     {

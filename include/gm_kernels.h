@@ -209,6 +209,12 @@ int gm_saxpy3(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct,
               const gm_csr *A, const gm_csr *B, const gm_semiring *sr,
               const gm_saxpy3_params *params);
 
+// *flops = multiplies of C=A*B without a mask: the sum over A's entries
+// (i,k) of |B(k,:)| (the total of GB_AxB_saxpy3_flopcount,
+// Source/mxm/GB_AxB_saxpy3_flopcount.c, taken over A's rows).  GrB_mxm uses
+// it to choose between saxpy3 and dot3 for C<M>=A*B.
+int gm_flopcount(int64_t *flops, const gm_csr *A, const gm_csr *B);
+
 //------------------------------------------------------------------------------
 // steps around the products: transpose, accumulate and mask
 //------------------------------------------------------------------------------

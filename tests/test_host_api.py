@@ -65,6 +65,24 @@ def test_semiring_decode():
         assert (s.add_op, s.mult_op, s.ztype) == trip, name
 
 
+def test_flop_rule_for_masked_saxpy():
+    """C<M>=A*B goes to dot3 on B' only when the flop count is well above
+    the dot3 work estimate nnz(M)*(mean |A(i,:)| + mean |B(:,j)|) + 16 nnz(B)."""
+    pref = capi.lib().GB_mxm_prefer_dot3
+    n = 1 << 20
+    # Erdos-Renyi 8 per row, mask 8 per row: 2^26 flops vs 2^23*16 + 16*2^23
+    assert not pref(8 * 8 * n, 8 * n, 8 * n, n, 8 * n, n)
+    # triangle count C<L>=L*L, L with 16 per row and heavy columns: 5e10 flops
+    assert pref(50_000_000_000, 16 * n, 16 * n, n, 16 * n, n)
+    # boundary: flops exactly 4x the estimate is not enough
+    est = 100 * (2.0 + 3.0) + 16 * 300
+    assert not pref(int(4 * est), 100, 200, 100, 300, 100)
+    assert pref(int(4 * est) + 1, 100, 200, 100, 300, 100)
+    # an empty mask or no flops: saxpy3
+    assert not pref(10 ** 12, 0, 10, 10, 10, 10)
+    assert not pref(0, 10, 10, 10, 10, 10)
+
+
 def test_method_selection():
     sel = capi.lib().GB_mxm_select
     # (mask, comp, a_tran, b_tran) -> 1 dot3, 2 saxpy3 (B' transposed first)
