@@ -78,6 +78,17 @@ __device__ __forceinline__ uint32_t hx_hash(uint32_t key, int lg)
     return (key * 0x9E3779B1u) >> (32 - lg);
 }
 
+// One row's summary for pair setup, read as one aligned 32-byte record:
+// start, length, first and last column, and its probe index (kind -1: none,
+// 0: bitmap, else log2 of the hash size) at unit offset off of the pool.
+// Built when the columns fit 31 bits.
+struct __align__(32) RowDesc
+{
+    int64_t p0;
+    int64_t off;
+    int32_t len, lo, hi, kind;
+};
+
 struct Dot3Args
 {
     const int64_t *Mp, *Mj;
@@ -100,6 +111,7 @@ struct Dot3Args
     uint64_t *bkeys;              // NULL, or 3 x mnz grouping keys of the binned entries
     const uint8_t *rowmode;       // NULL, or 1 per mask row handled whole by k_dot3_rows
     int64_t key_nA;               // probe-2 keys are key_nA + j
+    const RowDesc *dA, *dB;       // NULL, or one summary per row of A / B
 };
 
 // lane-aggregated append: returns the slot for lanes with pred set.
@@ -125,6 +137,7 @@ struct Pair
 {
     int64_t pa, pa_end, pb, pb_end;
     int64_t off, lo, hi;          // probe index of the longer row
+    int64_t slo, shi;             // first / last column of the probing (short) row
     int probe;
     int hlg;                      // 0: bitmap, else log2 of the hash size
     bool empty;                   // no possible common index
@@ -140,6 +153,37 @@ __device__ __forceinline__ int64_t d3_work_merge(int64_t a, int64_t b)
 // fill the pair for A(i,:) . B(j,:); returns the estimated work
 __device__ __forceinline__ int64_t d3_pair(const Dot3Args &g, int64_t i, int64_t j, Pair &P)
 {
+    if (g.dA)
+    {
+        const RowDesc da = g.dA[i], db = g.dB[j];
+        P.pa = da.p0; P.pa_end = da.p0 + da.len;
+        P.pb = db.p0; P.pb_end = db.p0 + db.len;
+        P.probe = 0;
+        P.empty = true;
+        const int64_t a = da.len, b = db.len;
+        if (a == 0 || b == 0) return 0;
+        if (da.hi < db.lo || db.hi < da.lo) return 0;
+        P.empty = false;
+        if (a >= b && da.kind >= 0)
+        {
+            P.probe = 1; P.off = da.off; P.lo = da.lo; P.hi = da.hi; P.hlg = da.kind;
+            P.slo = db.lo; P.shi = db.hi;
+            return b;
+        }
+        if (db.kind >= 0)
+        {
+            P.probe = 2; P.off = db.off; P.lo = db.lo; P.hi = db.hi; P.hlg = db.kind;
+            P.slo = da.lo; P.shi = da.hi;
+            return a;
+        }
+        if (da.kind >= 0)
+        {
+            P.probe = 1; P.off = da.off; P.lo = da.lo; P.hi = da.hi; P.hlg = da.kind;
+            P.slo = db.lo; P.shi = db.hi;
+            return b;
+        }
+        return d3_work_merge(a, b);
+    }
     P.pa = g.Ap[i]; P.pa_end = g.Ap[i + 1];
     P.pb = g.Bp[j]; P.pb_end = g.Bp[j + 1];
     P.probe = 0;
@@ -155,18 +199,21 @@ __device__ __forceinline__ int64_t d3_pair(const Dot3Args &g, int64_t i, int64_t
     {
         P.probe = 1; P.off = g.bA.off[i]; P.lo = alo; P.hi = ahi;
         P.hlg = bm_eligible(0, a, alo, ahi) ? 0 : hx_log2(a);
+        P.slo = blo; P.shi = bhi;
         return b;
     }
     if (b >= g.bB.T)
     {
         P.probe = 2; P.off = g.bB.off[j]; P.lo = blo; P.hi = bhi;
         P.hlg = bm_eligible(0, b, blo, bhi) ? 0 : hx_log2(b);
+        P.slo = alo; P.shi = ahi;
         return a;
     }
     if (a >= g.bA.T)
     {
         P.probe = 1; P.off = g.bA.off[i]; P.lo = alo; P.hi = ahi;
         P.hlg = bm_eligible(0, a, alo, ahi) ? 0 : hx_log2(a);
+        P.slo = blo; P.shi = bhi;
         return b;
     }
     return d3_work_merge(a, b);
@@ -235,7 +282,7 @@ __device__ __forceinline__ void d3_trim_short(const Dot3Args &g, const Pair &P, 
     const bool shortA = (P.probe == 2);
     const int64_t *Sj = shortA ? g.Aj : g.Bj;
     int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
-    const int64_t slo = Sj[s0], shi = Sj[s1 - 1];
+    const int64_t slo = P.slo, shi = P.shi;
     const int64_t olo = slo > P.lo ? slo : P.lo, ohi = shi < P.hi ? shi : P.hi;
     if (olo > slo) s0 = lower_bound(Sj, s0, s1, olo);
     if (ohi < shi) s1 = lower_bound(Sj, s0, s1, ohi + 1);
@@ -280,7 +327,7 @@ __device__ __forceinline__ void d3_trim_short_team(const Dot3Args &g, const Pair
     const bool shortA = (P.probe == 2);
     const int64_t *Sj = shortA ? g.Aj : g.Bj;
     int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
-    const int64_t slo = Sj[s0], shi = Sj[s1 - 1];
+    const int64_t slo = P.slo, shi = P.shi;
     const int64_t olo = slo > P.lo ? slo : P.lo, ohi = shi < P.hi ? shi : P.hi;
     if (olo > slo) s0 = team_lower_bound<G>(Sj, s0, s1, olo, lane);
     if (ohi < shi) s1 = team_lower_bound<G>(Sj, s0, s1, ohi + 1, lane);
@@ -786,6 +833,227 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint
 }
 
 //------------------------------------------------------------------------------
+// LDS probing of the heavy bins (pattern semirings)
+//------------------------------------------------------------------------------
+
+// In a bin sorted by probed row, consecutive entries read one long row.
+// The sorted bin is cut into fixed slices of LC_SLICE entries; in a slice, a
+// run of at least LC_GMIN entries with the same probed long row builds that
+// row's bitmap in LDS, LC_BITS columns at a time, from its sorted column
+// list (one streamed pass over the row, whatever index it has in HBM), and
+// each entry's short row (one wavefront per entry, 64 elements per load)
+// is tested against the chunk in LDS instead of with one random L2 access
+// per element.  An entry keeps its position in its short row and its
+// partial result in LDS from chunk to chunk; elements are visited in
+// ascending column order, so partials combine in k order.  Other entries
+// of the slice take the global path of the wavefront kernel.  Slices are
+// handed out per XCD region, as in the grouped team kernels, so a long
+// row's bitmap chunks are read from one L2.
+constexpr int LC_BITS = 1 << 19;              // columns per chunk: 64 KB of LDS
+constexpr int LC_WORDS = LC_BITS / 64;
+constexpr int LC_SLICE = 256;
+constexpr int LC_GMIN = 8;
+constexpr int LC_UNROLL = 8;
+constexpr int LC_NT = 512;                    // 8 wavefronts share a chunk
+
+template <class S>
+__global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *list,
+                                                    const uint64_t *keys, int64_t count,
+                                                    unsigned long long *ctr)
+{
+    using Z = typename S::Z;
+    static_assert(S::pattern, "LDS probing covers the pattern semirings");
+    constexpr int NW = LC_NT / 64;
+    __shared__ uint64_t s_bits[LC_WORDS];
+    __shared__ uint64_t s_key[LC_SLICE];
+    __shared__ int64_t s_cur[LC_SLICE], s_end[LC_SLICE], s_nx[LC_SLICE];
+    __shared__ Z s_z[LC_SLICE];
+    __shared__ uint32_t s_p[LC_SLICE];
+    __shared__ uint8_t s_st[LC_SLICE];        // bit 0: result exists, 1: LDS entry, 2: visited
+    __shared__ int64_t s_slice, s_lcur, s_lnext;
+    __shared__ int s_ge;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nslices = (count + LC_SLICE - 1) / LC_SLICE;
+    const int x0 = (int) (blockIdx.x & 7);
+    for (int k = 0; k < 8; k++)
+    {
+        const int r = (x0 + k) & 7;
+        const int64_t slo = nslices * r / 8, shi = nslices * (r + 1) / 8;
+        while (true)
+        {
+            if (threadIdx.x == 0) s_slice = slo + (int64_t) atomicAdd(&ctr[r], 1ull);
+            __syncthreads();
+            const int64_t sl = s_slice;
+            __syncthreads();
+            if (sl >= shi) break;
+            const int64_t q0 = sl * LC_SLICE;
+            const int n = (int) ((count - q0) < LC_SLICE ? (count - q0) : LC_SLICE);
+            for (int t = threadIdx.x; t < LC_SLICE; t += LC_NT) s_key[t] = t < n ? keys[q0 + t] : ~0ull;
+            __syncthreads();
+            int gs = 0;
+            while (gs < n)
+            {
+                // the run of entries with the slice's key at gs (keys are sorted)
+                if (threadIdx.x == 0) s_ge = n;
+                __syncthreads();
+                const uint64_t key = s_key[gs];
+                for (int t = gs + 1 + threadIdx.x; t < n; t += LC_NT)
+                    if (s_key[t] != key) atomicMin(&s_ge, t);
+                __syncthreads();
+                const int ge = s_ge;
+                const bool rowA = key < (uint64_t) g.key_nA;
+                const int64_t rr = rowA ? (int64_t) key : (int64_t) key - g.key_nA;
+                const RowDesc d = rowA ? g.dA[rr] : g.dB[rr];
+                const bool use_lds = (ge - gs >= LC_GMIN) && d.len > 0;
+                for (int t = gs + threadIdx.x; t < ge; t += LC_NT)
+                {
+                    const uint64_t w = list[q0 + t];
+                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+                    uint8_t st = 0;
+                    if (use_lds)
+                    {
+                        Pair P;
+                        (void) d3_pair(g, i, g.Mj[p], P);
+                        if (!P.empty && P.probe == (rowA ? 1 : 2))
+                        {
+                            st = 2;
+                            s_cur[t] = rowA ? P.pb : P.pa;
+                            s_end[t] = rowA ? P.pb_end : P.pa_end;
+                            s_nx[t] = P.slo;
+                            s_z[t] = S::identity();
+                        }
+                    }
+                    s_p[t] = (uint32_t) p;
+                    s_st[t] = st;
+                }
+                __syncthreads();
+                // entries on the global path: one wavefront each
+                for (int t = gs + wv; t < ge; t += NW)
+                {
+                    if (s_st[t] != 0) continue;
+                    const uint64_t w = list[q0 + t];
+                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+                    Pair P;
+                    (void) d3_pair(g, i, g.Mj[p], P);
+                    bool e;
+                    Z z;
+                    dot_lane<S, 64>(g, P, lane, e, z);
+                    team_reduce<S, 64>(e, z);
+                    if (lane == 0) d3_store<S>(g, p, e, z);
+                }
+                if (use_lds)
+                {
+                    const int64_t *Sj = rowA ? g.Bj : g.Aj;
+                    const int64_t *Lj = rowA ? g.Aj : g.Bj;
+                    const int64_t lend = d.p0 + d.len;
+                    const int64_t nwords = (((int64_t) d.hi - (int64_t) d.lo) >> 6) + 1;
+                    if (threadIdx.x == 0) s_lcur = d.p0;
+                    for (int64_t c0 = 0; c0 < nwords; c0 += LC_WORDS)
+                    {
+                        const int64_t cw = (nwords - c0) < LC_WORDS ? (nwords - c0) : LC_WORDS;
+                        const int64_t cbeg = (int64_t) d.lo + c0 * 64, cend = cbeg + cw * 64;
+                        __syncthreads();
+                        for (int64_t t = threadIdx.x; t < cw; t += LC_NT) s_bits[t] = 0;
+                        if (threadIdx.x == 0) s_lnext = lend;
+                        __syncthreads();
+                        // the long row's columns in [cbeg, cend): a contiguous run
+                        // from s_lcur on; each thread stops at its first column
+                        // past the chunk, the smallest such index starts the next
+                        const int64_t lc = s_lcur;
+                        for (int64_t e = lc + threadIdx.x; e < lend; e += LC_NT)
+                        {
+                            const int64_t x = Lj[e];
+                            if (x >= cend)
+                            {
+                                atomicMin((unsigned long long *) &s_lnext, (unsigned long long) e);
+                                break;
+                            }
+                            const int64_t dd = x - cbeg;
+                            atomicOr((unsigned long long *) &s_bits[dd >> 6], 1ull << (dd & 63));
+                        }
+                        __syncthreads();
+                        if (threadIdx.x == 0) s_lcur = s_lnext;
+                        for (int t = gs + wv; t < ge; t += NW)
+                        {
+                            const uint8_t st = s_st[t];
+                            if (!(st & 2) || s_nx[t] >= cend) continue;
+                            int64_t cur = s_cur[t];
+                            const int64_t end = s_end[t];
+                            // first visit: skip the short row's elements below the long row
+                            if (!(st & 4) && s_nx[t] < (int64_t) d.lo)
+                                cur = team_lower_bound<64>(Sj, cur, end, (int64_t) d.lo, lane);
+                            bool e = false;
+                            Z z = S::identity();
+                            int64_t nx = INT64_MAX;
+                            // LC_UNROLL windows of 64 elements per step, their
+                            // loads issued together (elements past the chunk are
+                            // read again on the next visit, mostly from cache)
+                            while (cur < end)
+                            {
+                                int64_t xs[LC_UNROLL];
+                                #pragma unroll
+                                for (int u = 0; u < LC_UNROLL; u++)
+                                {
+                                    const int64_t q = cur + u * 64 + lane;
+                                    xs[u] = q < end ? Sj[q] : INT64_MAX;
+                                }
+                                int taken = 0;
+                                bool stop = false;
+                                #pragma unroll
+                                for (int u = 0; u < LC_UNROLL; u++)
+                                {
+                                    const int64_t x = xs[u];
+                                    const bool in = x < cend;
+                                    if (in)
+                                    {
+                                        const int64_t dd = x - cbeg;
+                                        if ((s_bits[dd >> 6] >> (dd & 63)) & 1ull)
+                                        {
+                                            const Z tz = S::mul(Z(1), Z(1), x);
+                                            if (!e) { z = tz; e = true; }
+                                            else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                                            else z = S::upd(z, tz);
+                                        }
+                                    }
+                                    const int nin = __popcll(__ballot(in));
+                                    if (!stop)
+                                    {
+                                        taken += nin;
+                                        if (nin < 64)
+                                        {
+                                            nx = __shfl(x, nin, 64);
+                                            stop = true;
+                                        }
+                                    }
+                                }
+                                cur += taken;
+                                if (stop) break;
+                            }
+                            team_reduce<S, 64>(e, z);
+                            if (lane == 0)
+                            {
+                                bool ee = (st & 1) != 0;
+                                Z zz = s_z[t];
+                                combine<S>(ee, zz, e, z);
+                                s_z[t] = zz;
+                                s_st[t] = (uint8_t) ((st | 4) & ~1) | (ee ? 1 : 0);
+                                s_cur[t] = cur;
+                                s_nx[t] = cur < end ? nx : INT64_MAX;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    for (int t = gs + threadIdx.x; t < ge; t += LC_NT)
+                        if (s_st[t] & 2) d3_store<S>(g, (int64_t) s_p[t], (s_st[t] & 1) != 0, s_z[t]);
+                }
+                __syncthreads();
+                gs = ge;
+            }
+        }
+    }
+}
+
+//------------------------------------------------------------------------------
 // row mode: whole mask rows against an LDS bitmap of A(i,:)
 //------------------------------------------------------------------------------
 
@@ -1130,6 +1398,33 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_build(const int64_t *__restrict
     }
 }
 
+// one summary per row (off: the probe index offsets, or NULL when the
+// operand has none; rows of length >= T have an index)
+__global__ __launch_bounds__(D3_NT) void k_d3_desc(const int64_t *__restrict__ Xp,
+                                                   const int64_t *__restrict__ Xj, int64_t nrows,
+                                                   int64_t T, const int64_t *__restrict__ off,
+                                                   RowDesc *__restrict__ out)
+{
+    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
+         r += (int64_t) gridDim.x * D3_NT)
+    {
+        const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
+        RowDesc d;
+        d.p0 = p0;
+        d.off = 0;
+        d.len = (int32_t) len;
+        d.lo = len > 0 ? (int32_t) Xj[p0] : 0;
+        d.hi = len > 0 ? (int32_t) Xj[p0 + len - 1] : -1;
+        d.kind = -1;
+        if (off && len >= T)
+        {
+            d.off = off[r];
+            d.kind = bm_eligible(0, len, d.lo, d.hi) ? 0 : hx_log2(len);
+        }
+        out[r] = d;
+    }
+}
+
 static inline unsigned grid_for(int64_t work_items, int64_t per_block, int64_t cap)
 {
     int64_t g = (work_items + per_block - 1) / per_block;
@@ -1241,7 +1536,7 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
     while (end_bit < 64 && ((int64_t) 1 << end_bit) <= nkeys) end_bit++;
     DevBuf tk, tv, tmp, ctr;
     if (!tk.alloc(sizeof(uint64_t) * big) || !tv.alloc(sizeof(uint64_t) * big) ||
-        !ctr.alloc(24 * sizeof(unsigned long long)))
+        !ctr.alloc(40 * sizeof(unsigned long long)))
         return set_error(GM_OUT_OF_MEMORY, "dot3: grouping buffers");
     size_t tbytes = 0;
     for (int b = 0; b < 3; b++)
@@ -1253,7 +1548,10 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
         if (tb > tbytes) tbytes = tb;
     }
     if (!tmp.alloc(tbytes)) return set_error(GM_OUT_OF_MEMORY, "dot3: grouping sort");
-    k_zero_u64<<<1, 64, 0, st>>>(ctr.as<unsigned long long>(), 24);
+    k_zero_u64<<<1, 64, 0, st>>>(ctr.as<unsigned long long>(), 40);
+    // the wavefront and workgroup bins probe bitmaps from LDS when the
+    // semiring needs no positions and the row summaries exist
+    const bool lds = S::pattern && g.dA != nullptr && g.bA.words != nullptr;
     for (int b = 0; b < 3; b++)
     {
         if (cnt[b] == 0) continue;
@@ -1268,10 +1566,22 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
         }
         if (b == 0)
         {
+            // short rows: LDS chunks cost more than they save here
             ProfScope ps("dot3_team16");
             k_dot3_team_grp<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
                 g, tv.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
             GM_HIP(hipGetLastError(), "k_dot3_team_grp16");
+        }
+        else if (lds)
+        {
+            if constexpr (S::pattern)
+            {
+                ProfScope ps(b == 1 ? "dot3_wave_lds" : "dot3_block_lds");
+                k_dot3_lds<S><<<(unsigned) (cu_count() * 2), LC_NT, 0, st>>>(
+                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b],
+                    ctr.as<unsigned long long>() + (b == 1 ? 24 : 32));
+                GM_HIP(hipGetLastError(), "k_dot3_lds");
+            }
         }
         else if (b == 1)
         {
@@ -1359,6 +1669,27 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         if (minlen > 0)
             GM_TRY(dot3_bitmaps(g, A, B, !S::pattern, minlen, bm_offA, bm_offB, bm_words, bm_rank,
                                 bm_list, bm_ctr));
+        // row summaries for pair setup (columns must fit 31 bits)
+        g.dA = nullptr;
+        g.dB = nullptr;
+        DevBuf descA, descB;
+        if (A->ncols < ((int64_t) 1 << 31))
+        {
+            const bool shared = (A->p == B->p && A->j == B->j && A->nrows == B->nrows);
+            if (descA.alloc(sizeof(RowDesc) * (A->nrows > 0 ? A->nrows : 1)) &&
+                (shared || descB.alloc(sizeof(RowDesc) * (B->nrows > 0 ? B->nrows : 1))))
+            {
+                ProfScope ps("dot3_desc");
+                k_d3_desc<<<grid_for(A->nrows, D3_NT, cap), D3_NT, 0, st>>>(
+                    A->p, A->j, A->nrows, g.bA.T, g.bA.off, descA.as<RowDesc>());
+                if (!shared)
+                    k_d3_desc<<<grid_for(B->nrows, D3_NT, cap), D3_NT, 0, st>>>(
+                        B->p, B->j, B->nrows, g.bB.T, g.bB.off, descB.as<RowDesc>());
+                GM_HIP(hipGetLastError(), "k_d3_desc");
+                g.dA = descA.as<RowDesc>();
+                g.dB = shared ? g.dA : descB.as<RowDesc>();
+            }
+        }
         k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
         // whole rows to the row kernel: mask rows with at least row_min
         // entries whose A row spans fewer than RM_BITS columns

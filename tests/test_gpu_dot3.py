@@ -173,6 +173,40 @@ def test_dot3_grouped_tricount_default(grb):
     assert_csr_equal(got, ref, what="tricount scale 17")
 
 
+def _wide_rows(m, n, long_rows, long_len, short_len, rng):
+    """m rows over n columns: rows in long_rows have about long_len sorted
+    distinct columns spread over [0, n), the others about short_len; half of
+    each row comes from 4000 shared columns."""
+    hot = np.random.default_rng(7).choice(n, 4000, replace=False)   # shared by all rows
+    p = [0]
+    js = []
+    for r in range(m):
+        k = long_len if r in long_rows else short_len
+        row = np.unique(np.concatenate([rng.choice(hot, k // 2, replace=False),
+                                        rng.choice(n, k - k // 2, replace=False)]))
+        js.append(row.astype(np.int64))
+        p.append(p[-1] + len(row))
+    return W.Csr(m, n, np.array(p, dtype=np.int64), np.concatenate(js), None)
+
+
+@pytest.mark.parametrize("wave_max", [100000, 3])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GxB_PLUS_PAIR_INT32", "GxB_ANY_PAIR_BOOL",
+                                    "GxB_ANY_SECONDI_INT64"])
+def test_dot3_lds_chunks(grb, wave_max, handle):
+    """Heavy bins grouped by probed row, with long rows spanning several LDS
+    chunks of 2^19 columns (the LDS-probe kernel), against the oracle."""
+    print(f"[dot3] lds chunks wave_max={wave_max} {handle}", flush=True)
+    rng = np.random.default_rng(hash((wave_max, handle)) % 2 ** 32)
+    n = 1_500_000
+    A = _wide_rows(240, n, set(range(0, 240, 12)), 3000, 150, rng)
+    B = _wide_rows(200, n, set(range(3, 200, 10)), 2500, 120, rng)
+    M = rand_csr(240, 200, 0.35, rng)
+    name, _ = SR[handle]
+    ref = O.dot3(name, M, A, B)
+    got = gm_dot3(handle, M, A, B, params=(1, 2, wave_max, 8, 2, -1))
+    assert_csr_equal(got, ref, what=f"lds chunks {wave_max}")
+
+
 @pytest.mark.parametrize("row_min", [1, 4])
 @pytest.mark.parametrize("bitmaps", [-1, 8])
 @pytest.mark.parametrize("handle", list(SR))
