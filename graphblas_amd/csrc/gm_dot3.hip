@@ -1053,6 +1053,137 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
     }
 }
 
+// Single-chunk form for operands whose rows all span at most L1_BITS
+// columns (small graphs, e.g. a batch of them): a run of at least L1_GMIN
+// entries of a sorted bin that probe the same long row builds that row's
+// whole bitmap in LDS once (plus, for valued semirings, the row position of
+// the first entry of every 64-column word), and teams of G lanes probe their
+// short rows against it: lane l takes elements l, l+G, ... (four loads in
+// flight) and the partials combine in lane order, as in the global probe.
+// Other entries run the team kernel's own path.
+constexpr int L1_BITS = 1 << 16;
+constexpr int L1_WORDS = L1_BITS / 64;
+constexpr int L1_SLICE = 256;
+constexpr int L1_GMIN = 4;
+
+template <class S, int G>
+__global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t *list,
+                                                     const uint64_t *keys, int64_t count,
+                                                     unsigned long long *ctr)
+{
+    using Z = typename S::Z;
+    constexpr int TPB = D3_NT / G;
+    constexpr bool RANK = !S::pattern;
+    __shared__ uint64_t s_bits[L1_WORDS];
+    __shared__ uint32_t s_rank[RANK ? L1_WORDS : 1];
+    __shared__ uint32_t s_lists[TPB][StageCap<G>::value];
+    __shared__ uint64_t s_key[L1_SLICE];
+    __shared__ int64_t s_slice;
+    __shared__ int s_ge;
+    const int lane = threadIdx.x & (G - 1), team = threadIdx.x / G;
+    uint32_t *sl = s_lists[team];
+    const Z *Ax = (const Z *) g.Ax;
+    const Z *Bx = (const Z *) g.Bx;
+    const int64_t nslices = (count + L1_SLICE - 1) / L1_SLICE;
+    const int x0 = (int) (blockIdx.x & 7);
+    for (int k = 0; k < 8; k++)
+    {
+        const int r = (x0 + k) & 7;
+        const int64_t slo = nslices * r / 8, shi = nslices * (r + 1) / 8;
+        while (true)
+        {
+            if (threadIdx.x == 0) s_slice = slo + (int64_t) atomicAdd(&ctr[r], 1ull);
+            __syncthreads();
+            const int64_t sl_id = s_slice;
+            __syncthreads();
+            if (sl_id >= shi) break;
+            const int64_t q0 = sl_id * L1_SLICE;
+            const int n = (int) ((count - q0) < L1_SLICE ? (count - q0) : L1_SLICE);
+            for (int t = threadIdx.x; t < L1_SLICE; t += D3_NT) s_key[t] = t < n ? keys[q0 + t] : ~0ull;
+            __syncthreads();
+            int gs = 0;
+            while (gs < n)
+            {
+                if (threadIdx.x == 0) s_ge = n;
+                __syncthreads();
+                const uint64_t key = s_key[gs];
+                for (int t = gs + 1 + threadIdx.x; t < n; t += D3_NT)
+                    if (s_key[t] != key) atomicMin(&s_ge, t);
+                __syncthreads();
+                const int ge = s_ge;
+                const bool rowA = key < (uint64_t) g.key_nA;
+                const int64_t rr = rowA ? (int64_t) key : (int64_t) key - g.key_nA;
+                const RowDesc d = rowA ? g.dA[rr] : g.dB[rr];
+                const int64_t nwords = (((int64_t) d.hi - (int64_t) d.lo) >> 6) + 1;
+                const bool use_lds = (ge - gs >= L1_GMIN) && d.len > 0 && nwords <= L1_WORDS;
+                if (use_lds)
+                {
+                    const int64_t *Lj = rowA ? g.Aj : g.Bj;
+                    for (int t = threadIdx.x; t < nwords; t += D3_NT) s_bits[t] = 0;
+                    __syncthreads();
+                    for (int64_t e = d.p0 + threadIdx.x; e < d.p0 + d.len; e += D3_NT)
+                    {
+                        const int64_t dd = Lj[e] - d.lo;
+                        const int64_t wd = dd >> 6;
+                        atomicOr((unsigned long long *) &s_bits[wd], 1ull << (dd & 63));
+                        if constexpr (RANK)
+                        {
+                            // the first entry of its word records its position
+                            if (e == d.p0 || ((Lj[e - 1] - d.lo) >> 6) != wd)
+                                s_rank[wd] = (uint32_t) (e - d.p0);
+                        }
+                    }
+                    __syncthreads();
+                }
+                for (int t = gs + team; t < ge; t += TPB)
+                {
+                    const uint64_t w = list[q0 + t];
+                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+                    Pair P;
+                    (void) d3_pair(g, i, g.Mj[p], P);
+                    if (!(use_lds && !P.empty && P.probe == (rowA ? 1 : 2)))
+                    {
+                        team_task<S, G>(g, w, lane, sl);
+                        continue;
+                    }
+                    // the short row against the LDS bitmap of the long one
+                    const bool shortA = !rowA;
+                    const int64_t *Sj = shortA ? g.Aj : g.Bj;
+                    const int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
+                    bool e = false;
+                    Z z = S::identity();
+                    #pragma unroll 4
+                    for (int64_t q = s0 + lane; q < s1; q += G)
+                    {
+                        const int64_t x = Sj[q];
+                        if (x < (int64_t) d.lo || x > (int64_t) d.hi) continue;
+                        const int64_t dd = x - d.lo;
+                        const uint64_t word = s_bits[dd >> 6];
+                        const uint64_t bit = 1ull << (dd & 63);
+                        if (!(word & bit)) continue;
+                        Z tz;
+                        if constexpr (S::pattern)
+                            tz = S::mul(Z(1), Z(1), x);
+                        else
+                        {
+                            const int64_t pos = d.p0 + (int64_t) s_rank[dd >> 6] + __popcll(word & (bit - 1));
+                            const int64_t qa = shortA ? q : pos, qb = shortA ? pos : q;
+                            tz = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                        }
+                        if (!e) { z = tz; e = true; }
+                        else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                        else z = S::upd(z, tz);
+                    }
+                    team_reduce<S, G>(e, z);
+                    if (lane == 0) d3_store<S>(g, p, e, z);
+                }
+                __syncthreads();
+                gs = ge;
+            }
+        }
+    }
+}
+
 //------------------------------------------------------------------------------
 // row mode: whole mask rows against an LDS bitmap of A(i,:)
 //------------------------------------------------------------------------------
@@ -1403,11 +1534,15 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_build(const int64_t *__restrict
 __global__ __launch_bounds__(D3_NT) void k_d3_desc(const int64_t *__restrict__ Xp,
                                                    const int64_t *__restrict__ Xj, int64_t nrows,
                                                    int64_t T, const int64_t *__restrict__ off,
-                                                   RowDesc *__restrict__ out)
+                                                   RowDesc *__restrict__ out,
+                                                   unsigned long long *maxspan)
 {
-    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
-         r += (int64_t) gridDim.x * D3_NT)
+    unsigned long long ms = 0;
+    // every lane of a wavefront runs the same number of iterations
+    for (int64_t r0 = (int64_t) blockIdx.x * D3_NT; r0 < nrows; r0 += (int64_t) gridDim.x * D3_NT)
     {
+        const int64_t r = r0 + threadIdx.x;
+        if (r >= nrows) continue;
         const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
         RowDesc d;
         d.p0 = p0;
@@ -1422,7 +1557,20 @@ __global__ __launch_bounds__(D3_NT) void k_d3_desc(const int64_t *__restrict__ X
             d.kind = bm_eligible(0, len, d.lo, d.hi) ? 0 : hx_log2(len);
         }
         out[r] = d;
+        const unsigned long long sp = len > 0 ? (unsigned long long) (d.hi - d.lo + 1) : 0ull;
+        ms = sp > ms ? sp : ms;
     }
+    // largest column span of a row: wavefront maximum, then an atomic only
+    // where it raises the value seen so far
+    #pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const unsigned long long t = __shfl_xor(ms, o, 64);
+        ms = t > ms ? t : ms;
+    }
+    if ((threadIdx.x & 63) == 0 &&
+        ms > __hip_atomic_load(maxspan, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(maxspan, ms);
 }
 
 static inline unsigned grid_for(int64_t work_items, int64_t per_block, int64_t cap)
@@ -1525,10 +1673,12 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
     hipStream_t st = stream();
     unsigned long long *hc = (unsigned long long *) pinned_scratch(4 * sizeof(unsigned long long));
     if (!hc) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
-    GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+    GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
            "bin counts");
     GM_HIP(hipStreamSynchronize(st), "bin counts sync");
     const int64_t cnt[3] = {(int64_t) hc[0], (int64_t) hc[1], (int64_t) hc[2]};
+    // every row spans at most L1_BITS columns: whole bitmaps fit in LDS
+    const bool small = g.dA != nullptr && hc[3] > 0 && hc[3] <= (unsigned long long) L1_BITS;
     int64_t big = cnt[0] > cnt[1] ? cnt[0] : cnt[1];
     if (cnt[2] > big) big = cnt[2];
     if (big == 0) return GM_SUCCESS;
@@ -1564,7 +1714,18 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
                                              tv.as<uint64_t>(), (size_t) cnt[b], 0, (unsigned) end_bit, st),
                    "grouping sort");
         }
-        if (b == 0)
+        if (b < 2 && small)
+        {
+            ProfScope ps(b == 0 ? "dot3_team16_lds1" : "dot3_wave_lds1");
+            if (b == 0)
+                k_dot3_lds1<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
+            else
+                k_dot3_lds1<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
+            GM_HIP(hipGetLastError(), "k_dot3_lds1");
+        }
+        else if (b == 0)
         {
             // short rows: LDS chunks cost more than they save here
             ProfScope ps("dot3_team16");
@@ -1669,7 +1830,9 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         if (minlen > 0)
             GM_TRY(dot3_bitmaps(g, A, B, !S::pattern, minlen, bm_offA, bm_offB, bm_words, bm_rank,
                                 bm_list, bm_ctr));
-        // row summaries for pair setup (columns must fit 31 bits)
+        k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
+        // row summaries for pair setup (columns must fit 31 bits); bin_cnt[3]
+        // receives the largest column span of a row
         g.dA = nullptr;
         g.dB = nullptr;
         DevBuf descA, descB;
@@ -1681,16 +1844,15 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
             {
                 ProfScope ps("dot3_desc");
                 k_d3_desc<<<grid_for(A->nrows, D3_NT, cap), D3_NT, 0, st>>>(
-                    A->p, A->j, A->nrows, g.bA.T, g.bA.off, descA.as<RowDesc>());
+                    A->p, A->j, A->nrows, g.bA.T, g.bA.off, descA.as<RowDesc>(), g.bin_cnt + 3);
                 if (!shared)
                     k_d3_desc<<<grid_for(B->nrows, D3_NT, cap), D3_NT, 0, st>>>(
-                        B->p, B->j, B->nrows, g.bB.T, g.bB.off, descB.as<RowDesc>());
+                        B->p, B->j, B->nrows, g.bB.T, g.bB.off, descB.as<RowDesc>(), g.bin_cnt + 3);
                 GM_HIP(hipGetLastError(), "k_d3_desc");
                 g.dA = descA.as<RowDesc>();
                 g.dB = shared ? g.dA : descB.as<RowDesc>();
             }
         }
-        k_zero_u64<<<1, 64, 0, st>>>(g.bin_cnt, 4);
         // whole rows to the row kernel: mask rows with at least row_min
         // entries whose A row spans fewer than RM_BITS columns
         const int64_t row_min = (prm && prm->row_min != 0) ? prm->row_min

@@ -207,6 +207,26 @@ def test_dot3_lds_chunks(grb, wave_max, handle):
     assert_csr_equal(got, ref, what=f"lds chunks {wave_max}")
 
 
+@pytest.mark.parametrize("params", [(1, 64, 100000, 8), (8, 512, 4096, 128), (1, 2, 100000, 16)])
+@pytest.mark.parametrize("handle", list(SR))
+def test_dot3_lds_whole_rows(grb, params, handle):
+    """Every row spans at most 2^16 columns: the grouped team16 and wavefront
+    bins probe whole long-row bitmaps (with word ranks for valued semirings)
+    from LDS.  Against the oracle, structural and valued masks."""
+    print(f"[dot3] lds whole rows {params} {handle}", flush=True)
+    rng = np.random.default_rng(hash((params, handle)) % 2 ** 32)
+    vt = _vals(handle)
+    A = power_law_csr(3000, 40, rng, values=vt)
+    B = power_law_csr(3000, 40, rng, values=vt)
+    M = power_law_csr(3000, 30, rng, values="int64")
+    name, _ = SR[handle]
+    for struct in (True, False):
+        ref = O.dot3(name, M, A, B, mask_struct=struct)
+        got = gm_dot3(handle, M, A, B, mask_struct=struct, params=tuple(params) + (2, -1))
+        assert_csr_equal(got, ref, fp_tol=FP_TOL if "fp64" in name else None,
+                         what=f"lds whole rows struct={struct}")
+
+
 @pytest.mark.parametrize("row_min", [1, 4])
 @pytest.mark.parametrize("bitmaps", [-1, 8])
 @pytest.mark.parametrize("handle", list(SR))
