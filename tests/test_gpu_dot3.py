@@ -189,22 +189,24 @@ def _wide_rows(m, n, long_rows, long_len, short_len, rng):
     return W.Csr(m, n, np.array(p, dtype=np.int64), np.concatenate(js), None)
 
 
-@pytest.mark.parametrize("wave_max", [100000, 3])
+@pytest.mark.parametrize("bins", [(2, 100000), (2, 3), (100000, 100000)])
 @pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GxB_PLUS_PAIR_INT32", "GxB_ANY_PAIR_BOOL",
                                     "GxB_ANY_SECONDI_INT64"])
-def test_dot3_lds_chunks(grb, wave_max, handle):
-    """Heavy bins grouped by probed row, with long rows spanning several LDS
-    chunks of 2^19 columns (the LDS-probe kernel), against the oracle."""
-    print(f"[dot3] lds chunks wave_max={wave_max} {handle}", flush=True)
-    rng = np.random.default_rng(hash((wave_max, handle)) % 2 ** 32)
+def test_dot3_lds_chunks(grb, bins, handle):
+    """Grouped bins with long rows spanning several LDS chunks of 2^19
+    columns: the wavefront and workgroup bins (chunked LDS bitmaps) and the
+    team16 bin (global probes), against the oracle."""
+    team16_max, wave_max = bins
+    print(f"[dot3] lds chunks {bins} {handle}", flush=True)
+    rng = np.random.default_rng(hash((bins, handle)) % 2 ** 32)
     n = 1_500_000
     A = _wide_rows(240, n, set(range(0, 240, 12)), 3000, 150, rng)
     B = _wide_rows(200, n, set(range(3, 200, 10)), 2500, 120, rng)
     M = rand_csr(240, 200, 0.35, rng)
     name, _ = SR[handle]
     ref = O.dot3(name, M, A, B)
-    got = gm_dot3(handle, M, A, B, params=(1, 2, wave_max, 8, 2, -1))
-    assert_csr_equal(got, ref, what=f"lds chunks {wave_max}")
+    got = gm_dot3(handle, M, A, B, params=(1, team16_max, wave_max, 8, 2, -1))
+    assert_csr_equal(got, ref, what=f"lds chunks {bins}")
 
 
 @pytest.mark.parametrize("params", [(1, 64, 100000, 8), (8, 512, 4096, 128), (1, 2, 100000, 16)])
