@@ -194,6 +194,7 @@ def lib():
                            ctypes.c_int], ctypes.c_int),
         "GB_semiring_codes": ([P, ctypes.POINTER(GmSemiring)], ctypes.c_bool),
         "GB_mxm_prefer_dot3": ([ctypes.c_int64] * 6, ctypes.c_bool),
+        "GB_AxB_dot2_control": ([ctypes.c_double] * 5, ctypes.c_bool),
         # kernel layer
         "gm_init": ([], ctypes.c_int),
         "gm_finalize": ([], None),
@@ -259,6 +260,10 @@ def lib():
         "gm_transpose": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr)], ctypes.c_int),
         "gm_flopcount": ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr)],
                          ctypes.c_int),
+        "gm_nonempty_rows": ([ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(GmCsr)], ctypes.c_int),
+        "gm_dot2": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.c_bool, ctypes.c_bool,
+                     ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
+                     ctypes.POINTER(GmDot3Params)], ctypes.c_int),
         "gm_accum_mask": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                            ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr),
                            I32], ctypes.c_int),

@@ -1956,6 +1956,158 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
 
 using namespace gm;
 
+
+//------------------------------------------------------------------------------
+// dot2: C = A*B' or C<!M> = A*B' with every dot product of the small result
+// computed (the reference's bitmap dot product, Source/mxm/GB_AxB_dot2.c).
+// Here every pair (i, j) of a nonempty row of A and a nonempty row of B that
+// the mask allows becomes an entry of a structural mask, and dot3 computes
+// them: the same dot products, in the same k order, and a compacted sparse C
+// in place of the reference's bitmap.
+//------------------------------------------------------------------------------
+
+// f[r] = 1 when row r is nonempty
+__global__ __launch_bounds__(D3_NT) void k_d2_nonempty(const int64_t *__restrict__ Xp, int64_t n,
+                                                       int64_t *__restrict__ f)
+{
+    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < n; r += (int64_t) gridDim.x * D3_NT)
+        f[r] = Xp[r + 1] > Xp[r] ? 1 : 0;
+}
+
+// list[pos[r]] = r for the nonempty rows
+__global__ __launch_bounds__(D3_NT) void k_d2_list(const int64_t *__restrict__ Xp, int64_t n,
+                                                   const int64_t *__restrict__ pos, int64_t *__restrict__ list)
+{
+    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < n; r += (int64_t) gridDim.x * D3_NT)
+        if (Xp[r + 1] > Xp[r]) list[pos[r]] = r;
+}
+
+// entries of the full mask in row i: all nonempty rows of B when A(i,:) is
+// nonempty, less the ones where M(i,j) is true (C<!M>)
+__global__ __launch_bounds__(D3_NT) void k_d2_count(const int64_t *__restrict__ Ap, int64_t m, int64_t bnvec,
+                                                    const int64_t *__restrict__ fB, const int64_t *Mp,
+                                                    const int64_t *Mj, const void *Mx, int32_t msize,
+                                                    int32_t m_iso, int32_t mask_struct, int64_t *__restrict__ cnt)
+{
+    for (int64_t i = (int64_t) blockIdx.x * D3_NT + threadIdx.x; i < m; i += (int64_t) gridDim.x * D3_NT)
+    {
+        int64_t c = 0;
+        if (Ap[i + 1] > Ap[i])
+        {
+            c = bnvec;
+            if (Mp)
+                for (int64_t e = Mp[i]; e < Mp[i + 1]; e++)
+                    if (fB[Mj[e]] && (mask_struct || mask_true(Mx, e, msize, m_iso))) c--;
+        }
+        cnt[i] = c;
+    }
+}
+
+// one wavefront per row: the row's list of allowed columns, in order
+__global__ __launch_bounds__(D3_NT) void k_d2_fill(const int64_t *__restrict__ Ap, int64_t m,
+                                                   const int64_t *__restrict__ listB, int64_t bnvec,
+                                                   const int64_t *Mp, const int64_t *Mj, const void *Mx,
+                                                   int32_t msize, int32_t m_iso, int32_t mask_struct,
+                                                   const int64_t *__restrict__ Fp, int64_t *__restrict__ Fj)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wv = ((int64_t) blockIdx.x * D3_NT + threadIdx.x) >> 6;
+    const int64_t nwv = ((int64_t) gridDim.x * D3_NT) >> 6;
+    for (int64_t i = wv; i < m; i += nwv)
+    {
+        if (Ap[i + 1] == Ap[i]) continue;
+        int64_t out = Fp[i];
+        const int64_t m0 = Mp ? Mp[i] : 0, m1 = Mp ? Mp[i + 1] : 0;
+        for (int64_t t0 = 0; t0 < bnvec; t0 += 64)
+        {
+            const int64_t t = t0 + lane;
+            bool keep = false;
+            int64_t j = 0;
+            if (t < bnvec)
+            {
+                j = listB[t];
+                keep = true;
+                if (m1 > m0)
+                {
+                    const int64_t q = lower_bound(Mj, m0, m1, j);
+                    if (q < m1 && Mj[q] == j && (mask_struct || mask_true(Mx, q, msize, m_iso))) keep = false;
+                }
+            }
+            const unsigned long long bal = __ballot(keep);
+            if (keep) Fj[out + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+            out += __popcll(bal);
+        }
+    }
+}
+
+template <class S>
+static int dot2_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct, const gm_csr *A,
+                    const gm_csr *B, const gm_dot3_params *prm)
+{
+    (void) mask_comp;
+    hipStream_t st = stream();
+    const int64_t m = A->nrows, nb = B->nrows;
+    const int64_t cap = (int64_t) cu_count() * 16;
+    DevBuf fA, fB, posB, listB, cnt, Fp, Fj;
+    if (!fA.alloc(sizeof(int64_t) * (m > 0 ? m : 1)) || !fB.alloc(sizeof(int64_t) * (nb > 0 ? nb : 1)) ||
+        !posB.alloc(sizeof(int64_t) * (nb + 1)) || !cnt.alloc(sizeof(int64_t) * (m > 0 ? m : 1)) ||
+        !Fp.alloc(sizeof(int64_t) * (m + 1)))
+        return set_error(GM_OUT_OF_MEMORY, "dot2: workspace");
+    if (nb > 0)
+    {
+        k_d2_nonempty<<<grid_for(nb, D3_NT, cap), D3_NT, 0, st>>>(B->p, nb, fB.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_d2_nonempty");
+    }
+    GM_TRY(scan_exclusive_i64(fB.as<int64_t>(), posB.as<int64_t>(), nb));
+    int64_t *hp = (int64_t *) pinned_scratch(2 * sizeof(int64_t));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(hp, posB.as<int64_t>() + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st), "bnvec");
+    GM_HIP(hipStreamSynchronize(st), "bnvec sync");
+    const int64_t bnvec = hp[0];
+    if (!listB.alloc(sizeof(int64_t) * (bnvec > 0 ? bnvec : 1)))
+        return set_error(GM_OUT_OF_MEMORY, "dot2: row list");
+    if (nb > 0)
+    {
+        k_d2_list<<<grid_for(nb, D3_NT, cap), D3_NT, 0, st>>>(B->p, nb, posB.as<int64_t>(), listB.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_d2_list");
+    }
+    const bool use_m = M != nullptr;
+    const void *Mx = (use_m && !mask_struct) ? M->x : nullptr;
+    const int32_t mstruct = (!use_m || mask_struct || !M->x) ? 1 : 0;
+    const int32_t msize = use_m ? (int32_t) type_size(M->type) : 0;
+    if (m > 0)
+    {
+        k_d2_count<<<grid_for(m, D3_NT, cap), D3_NT, 0, st>>>(
+            A->p, m, bnvec, fB.as<int64_t>(), use_m ? M->p : nullptr, use_m ? M->j : nullptr, Mx, msize,
+            use_m ? M->iso : 0, mstruct, cnt.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_d2_count");
+    }
+    GM_TRY(scan_exclusive_i64(cnt.as<int64_t>(), Fp.as<int64_t>(), m));
+    GM_HIP(hipMemcpyAsync(hp + 1, Fp.as<int64_t>() + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz");
+    GM_HIP(hipStreamSynchronize(st), "nnz sync");
+    const int64_t fnz = hp[1];
+    if (fnz >= ((int64_t) 1 << 32))
+        return set_error(GM_NO_VALUE, "dot2: %lld dot products exceed the mask index range", (long long) fnz);
+    if (!Fj.alloc(sizeof(int64_t) * (fnz > 0 ? fnz : 1)))
+        return set_error(GM_OUT_OF_MEMORY, "dot2: %lld dot products", (long long) fnz);
+    if (m > 0 && fnz > 0)
+    {
+        k_d2_fill<<<grid_for(m * 64, D3_NT, cap), D3_NT, 0, st>>>(
+            A->p, m, listB.as<int64_t>(), bnvec, use_m ? M->p : nullptr, use_m ? M->j : nullptr, Mx, msize,
+            use_m ? M->iso : 0, mstruct, Fp.as<int64_t>(), Fj.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_d2_fill");
+    }
+    gm_csr F;
+    F.nrows = m;
+    F.ncols = nb;
+    F.p = Fp.as<int64_t>();
+    F.j = Fj.as<int64_t>();
+    F.x = nullptr;
+    F.iso = 0;
+    F.type = GM_BOOL;
+    return dot3_run<S>(C, &F, true, A, B, prm);
+}
+
 extern "C" bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
                                const gm_csr *B, const gm_semiring *sr)
 {
@@ -1987,4 +2139,52 @@ extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const g
     GM_SEMIRING_LIST(GM_DOT3_CASE)
 #undef GM_DOT3_CASE
     return rc;
+}
+
+extern "C" int gm_nonempty_rows(int64_t *count, const gm_csr *A)
+{
+    if (!count || !A) return set_error(GM_INVALID_VALUE, "gm_nonempty_rows: NULL argument");
+    *count = 0;
+    if (A->nrows <= 0) return GM_SUCCESS;
+    if (!A->p) return set_error(GM_INVALID_VALUE, "gm_nonempty_rows: NULL row pointers");
+    GM_TRY(ensure_init());
+    DevBuf f, pos;
+    if (!f.alloc(sizeof(int64_t) * A->nrows) || !pos.alloc(sizeof(int64_t) * (A->nrows + 1)))
+        return set_error(GM_OUT_OF_MEMORY, "gm_nonempty_rows: workspace");
+    hipStream_t st = stream();
+    k_d2_nonempty<<<grid_for(A->nrows, D3_NT, (int64_t) cu_count() * 16), D3_NT, 0, st>>>(A->p, A->nrows,
+                                                                                         f.as<int64_t>());
+    GM_HIP(hipGetLastError(), "k_d2_nonempty");
+    GM_TRY(scan_exclusive_i64(f.as<int64_t>(), pos.as<int64_t>(), A->nrows));
+    int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(hp, pos.as<int64_t>() + A->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st),
+           "nonempty count");
+    GM_HIP(hipStreamSynchronize(st), "nonempty sync");
+    *count = hp[0];
+    return GM_SUCCESS;
+}
+
+extern "C" int gm_dot2(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct, const gm_csr *A,
+                       const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
+{
+    if (!C || !A || !B || !sr) return set_error(GM_INVALID_VALUE, "gm_dot2: NULL argument");
+    if (M && !mask_comp)
+        return set_error(GM_NO_VALUE, "gm_dot2: a mask that is not complemented goes to dot3");
+    if (!semiring_supported(sr) || !operand_ok(sr, A) || !operand_ok(sr, B))
+        return set_error(GM_NO_VALUE, "gm_dot2: case not handled by the GPU kernels");
+    if (A->ncols != B->ncols || (M && (M->nrows != A->nrows || M->ncols != B->nrows)))
+        return set_error(GM_INVALID_VALUE, "gm_dot2: dimensions");
+    if (A->nrows >= ((int64_t) 1 << 32)) return set_error(GM_NO_VALUE, "gm_dot2: nrows beyond 2^32");
+    GM_TRY(ensure_init());
+#define GM_DOT2_CASE(ADD, MULT, ZC, Z)                                                      \
+    if (sr->add_op == ADD && sr->mult_op == MULT && sr->ztype == ZC)                        \
+    {                                                                                       \
+        int rc = dot2_run<SR<ADD, MULT, Z>>(C, M, mask_comp, mask_struct, A, B, params);    \
+        if (rc == GM_SUCCESS) C->type = ZC;                                                 \
+        return rc;                                                                          \
+    }
+    GM_SEMIRING_LIST(GM_DOT2_CASE)
+#undef GM_DOT2_CASE
+    return set_error(GM_NO_VALUE, "gm_dot2: semiring not instantiated");
 }

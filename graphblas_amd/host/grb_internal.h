@@ -88,10 +88,11 @@ GrB_Info GB_matrix_view(gm_csr *v, GrB_Matrix A);
 GrB_Type GB_type_from_code(int32_t code);
 
 // method selection (pure host logic, exported for tests)
-typedef enum { GB_METHOD_NONE = 0, GB_METHOD_DOT3 = 1, GB_METHOD_SAXPY3 = 2 } GB_method;
+typedef enum { GB_METHOD_NONE = 0, GB_METHOD_DOT3 = 1, GB_METHOD_SAXPY3 = 2, GB_METHOD_DOT2 = 3 } GB_method;
 GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran, int axb);
 bool GB_mxm_prefer_dot3(int64_t flops, int64_t nnz_m, int64_t nnz_a, int64_t m, int64_t nnz_b,
                         int64_t n);
+bool GB_AxB_dot2_control(double anz, double bnz, double anvec, double bnvec, double avlen);
 bool GB_semiring_codes(const GrB_Semiring s, gm_semiring *out);
 
 #endif

@@ -48,6 +48,26 @@ bool GB_mxm_prefer_dot3(int64_t flops, int64_t nnz_m, int64_t nnz_a, int64_t m, 
     return (double) flops > 4.0 * dot_work;
 }
 
+// C = A*B' (or C<!M> = A*B') as every dot product of the small result:
+// the rule of the reference's GB_AxB_dot2_control
+// (Source/mxm/GB_AxB_dot2_control.c:37-92), with A's rows in place of its
+// vectors.  anvec / bnvec: nonempty rows of A and of B (the rows of B'
+// columns), avlen: the inner dimension.
+bool GB_AxB_dot2_control(double anz, double bnz, double anvec, double bnvec, double avlen)
+{
+    const double cnz = anvec * bnvec;                       // the pairs to compute
+    const double row_degree = anz / (avlen > 1 ? avlen : 1);
+    const double col_degree = anz / (anvec > 1 ? anvec : 1);
+    if (cnz > anz + bnz) return false;                      // C large: saxpy
+    if (anz + bnz > 10000 * cnz || cnz <= 100) return true;  // C tiny: dot
+    if (row_degree < 0.125 && col_degree > 1200) return true;
+    return false;
+}
+
+// dot2 pairs above this count go to saxpy3 even when GxB_AxB_DOT asks for a
+// dot product (the structural mask of the pairs would not fit)
+#define GB_DOT2_MAX_PAIRS ((double) (1ll << 30))
+
 // the gm_csr view of a device result (no copy)
 static gm_csr GB_view_of(const gm_csr_out *R)
 {
@@ -173,7 +193,11 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
         va = GB_view_of(&AT);
     }
-    if (method == GB_METHOD_SAXPY3 && Mask && !mask_comp && !b_tran && axb == GxB_DEFAULT)
+    // (not for an ANY monoid with a positional multiplier, whose witness k
+    // differs between the methods: the first match in a dot product, the
+    // last writer in saxpy3)
+    if (method == GB_METHOD_SAXPY3 && Mask && !mask_comp && !b_tran && axb == GxB_DEFAULT &&
+        !(sr.add_op == GM_ADD_ANY && sr.mult_op == GM_MULT_K))
     {
         gm_csr bt_shape = vb;       // B' by its dimensions only, for the dot3 case check
         bt_shape.nrows = vb.ncols;
@@ -196,7 +220,33 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
             }
         }
     }
-    if (b_tran != (method == GB_METHOD_DOT3))
+    if (method == GB_METHOD_SAXPY3 && (!Mask || mask_comp) && (b_tran || axb == GxB_AxB_DOT) &&
+        (axb == GxB_DEFAULT || axb == GxB_AxB_DOT))
+    {
+        // C = A*B' or C<!M> = A*B': every dot product of a small, dense
+        // result (GB_AxB_meta_adotb_control.c:57-92), or on request
+        int64_t anvec = 0, bnvec = 0;
+        rc = gm_nonempty_rows(&anvec, &va);
+        if (rc == GM_SUCCESS && b_tran) rc = gm_nonempty_rows(&bnvec, &vb);
+        if (!b_tran) bnvec = vb.ncols;      // B' rows: at most B's columns
+        if (rc != GM_SUCCESS)
+        {
+            GB_free_out(&AT);
+            return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
+        }
+        const double anz = (double) (a_tran ? AT.nvals : A->nvals), bnz = (double) B->nvals;
+        const double pairs = (double) anvec * (double) bnvec;
+        const bool dot2 = (axb == GxB_AxB_DOT) ? pairs <= GB_DOT2_MAX_PAIRS
+                        : GB_AxB_dot2_control(anz, bnz, (double) anvec, (double) bnvec, (double) va.ncols);
+        if (dot2)
+        {
+            if (GB_burble)
+                printf("burble: GrB_mxm: C%s=A*B' with %lld x %lld nonempty rows: dot2\n",
+                       Mask ? "<!M>" : "", (long long) anvec, (long long) bnvec);
+            method = GB_METHOD_DOT2;
+        }
+    }
+    if (b_tran != (method == GB_METHOD_DOT3 || method == GB_METHOD_DOT2))
     {
         // saxpy3 needs B by rows, dot3 needs the rows of B' (B's columns)
         if (GB_burble) printf("burble: GrB_mxm: transpose B (%lld entries)\n", (long long) B->nvals);
@@ -215,6 +265,10 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
                    mstruct ? "S:" : "", GB_sr_name(&sr, nm, sizeof(nm)),
                    (long long) Mask->nvals);
         rc = gm_dot3(&R, &vm, mstruct, &va, &vb, &sr, NULL);
+    }
+    else if (method == GB_METHOD_DOT2)
+    {
+        rc = gm_dot2(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, NULL);
     }
     else
     {

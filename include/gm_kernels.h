@@ -199,6 +199,19 @@ bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
 int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
             const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params);
 
+// dot2: C = A*B' or C<!M> = A*B' (M may be NULL; a mask that is not
+// complemented goes to gm_dot3) with every dot product of a nonempty row of A
+// and a nonempty row of B computed, for small, dense results: the reference's
+// GB_AxB_dot2 (Source/mxm/GB_AxB_dot2.c), chosen by GB_AxB_dot2_control
+// (Source/mxm/GB_AxB_dot2_control.c).  The pairs become a structural mask of
+// dot3, so C comes back sparse and compacted instead of as a bitmap.
+int gm_dot2(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct, const gm_csr *A,
+            const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params);
+
+// *count = rows of A with at least one entry (the reference's
+// GB_nvec_nonempty, Source/hyper/GB_nvec_nonempty.c:17), for the dot2 rule
+int gm_nonempty_rows(int64_t *count, const gm_csr *A);
+
 //------------------------------------------------------------------------------
 // saxpy3: C = A*B, C<M> = A*B, C<!M> = A*B (row-wise Gustavson / hash)
 //------------------------------------------------------------------------------

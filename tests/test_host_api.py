@@ -83,6 +83,18 @@ def test_flop_rule_for_masked_saxpy():
     assert not pref(0, 10, 10, 10, 10, 10)
 
 
+def test_dot2_rule():
+    """GB_AxB_dot2_control restated: (anz, bnz, anvec, bnvec, avlen)."""
+    ctl = capi.lib().GB_AxB_dot2_control
+    assert ctl(5000, 4000, 8, 10, 5000)                 # 80 pairs <= 100: C tiny
+    assert not ctl(5000, 4000, 100, 100, 5000)          # 10000 pairs > anz + bnz: C large
+    assert ctl(2e6, 1e6, 10, 20, 1e4)                   # anz + bnz > 10000 * cnz
+    assert not ctl(3000, 3000, 50, 60, 100)             # 3000 pairs, none of the rules
+    # A' implicit: row degree < 0.125 and column degree > 1200
+    assert ctl(130000, 200000, 100, 300, 2e6)
+    assert not ctl(130000, 200000, 100, 300, 1e6)       # row degree 0.13
+
+
 def test_method_selection():
     sel = capi.lib().GB_mxm_select
     # (mask, comp, a_tran, b_tran) -> 1 dot3, 2 saxpy3 (B' transposed first)
