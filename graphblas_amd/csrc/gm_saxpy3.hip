@@ -735,11 +735,14 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
-// Four waves per SIMD (<= 128 VGPRs) with 32-bit keys: the look-back of one
-// workgroup is hidden behind the other workgroups of the CU.  64-bit keys
-// double the key LDS, which then allows three.
+// Five waves per SIMD (<= 102 VGPRs, LDS <= 32 KB) with 32-bit keys: the
+// look-back of one workgroup is hidden behind the other workgroups of the
+// CU (measured: 2% faster than four waves even with a few spilled
+// registers).  Positional multipliers keep four (their k array needs the
+// LDS); 64-bit keys double the key LDS, which then allows three.
 template <class S, class K, int NT, bool PK>
-__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1024 : 768) / NT) void k_sx_fused(SxArgs g, int64_t fs,
+__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? (S::mult == GM_MULT_K ? 1024 : 1280) : 768) / NT)
+void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
                                                          int *err)
 {
@@ -751,7 +754,9 @@ __global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1024 : 768) / NT) void k_sx_f
     __shared__ K s_key[FUSED_ROWS][FMAX + 1];         // sort keys, then output columns (int32)
     __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
     __shared__ int32_t s_pre[FUSED_ROWS][TG + 1];
-    __shared__ int64_t s_k[FUSED_ROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
+    // the inner index k of each staged A entry: only positional multipliers use it
+    constexpr int KROWS = (S::mult == GM_MULT_K) ? FUSED_ROWS : 1;
+    __shared__ int64_t s_k[KROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
     __shared__ Z s_av[FUSED_ROWS][TG + 1];
     __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
     __shared__ int s_big[FUSED_ROWS];
@@ -796,7 +801,7 @@ __global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1024 : 768) / NT) void k_sx_f
                 const int64_t tot = __shfl(inc, TG - 1, TG);
                 if (f + tot > fs) { big = 1; break; }
                 s_pre[team][sub] = (int32_t) (inc - len);
-                s_k[team][sub] = k;
+                if constexpr (S::mult == GM_MULT_K) s_k[team][sub] = k;
                 s_bs[team][sub] = bs;
                 s_av[team][sub] = av;
                 const int nA = (a1 - e0) < TG ? (int) (a1 - e0) : TG;
@@ -831,7 +836,8 @@ __global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1024 : 768) / NT) void k_sx_f
                             if constexpr (!S::iso_out)
                             {
                                 if constexpr (!PK) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
-                                val[fq] = S::mul(s_av[team][lo], bv, s_k[team][lo]);
+                                val[fq] = S::mul(s_av[team][lo], bv,
+                                                 (S::mult == GM_MULT_K) ? s_k[KROWS > 1 ? team : 0][lo] : 0);
                             }
                             nallow++;
                         }
