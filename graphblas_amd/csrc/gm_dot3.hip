@@ -851,12 +851,12 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint
 // of the slice take the global path of the wavefront kernel.  Slices are
 // handed out per XCD region, as in the grouped team kernels, so a long
 // row's bitmap chunks are read from one L2.
-constexpr int LC_BITS = 1 << 19;              // columns per chunk: 64 KB of LDS
+constexpr int LC_BITS = 1 << 20;              // columns per chunk: 128 KB of LDS
 constexpr int LC_WORDS = LC_BITS / 64;
-constexpr int LC_SLICE = 256;
+constexpr int LC_SLICE = 512;
 constexpr int LC_GMIN = 8;
 constexpr int LC_UNROLL = 8;
-constexpr int LC_NT = 512;                    // 8 wavefronts share a chunk
+constexpr int LC_NT = 1024;                   // 16 wavefronts share a chunk (one workgroup per CU)
 
 template <class S>
 __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *list,
@@ -1740,7 +1740,7 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
             if constexpr (S::pattern)
             {
                 ProfScope ps(b == 1 ? "dot3_wave_lds" : "dot3_block_lds");
-                k_dot3_lds<S><<<(unsigned) (cu_count() * 2), LC_NT, 0, st>>>(
+                k_dot3_lds<S><<<(unsigned) (cu_count() * 1), LC_NT, 0, st>>>(
                     g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b],
                     ctr.as<unsigned long long>() + (b == 1 ? 24 : 32));
                 GM_HIP(hipGetLastError(), "k_dot3_lds");
