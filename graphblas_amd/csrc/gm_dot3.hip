@@ -264,6 +264,9 @@ __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P,
     for (int64_t q = q0; q < q1; q += step)
     {
         const int64_t x = Sj[q];
+        // the short row is sorted: past the long row's last column, so is
+        // every later element of this lane
+        if (x > P.hi) break;
         const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
         if (pos >= 0)
         {
@@ -329,10 +332,11 @@ __device__ __forceinline__ void d3_trim_short_team(const Dot3Args &g, const Pair
     const bool shortA = (P.probe == 2);
     const int64_t *Sj = shortA ? g.Aj : g.Bj;
     int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
-    const int64_t slo = P.slo, shi = P.shi;
-    const int64_t olo = slo > P.lo ? slo : P.lo, ohi = shi < P.hi ? shi : P.hi;
+    const int64_t slo = P.slo;
+    const int64_t olo = slo > P.lo ? slo : P.lo;
+    // only the start is searched: the probe loop stops at the first element
+    // past the long row's range
     if (olo > slo) s0 = team_lower_bound<G>(Sj, s0, s1, olo, lane);
-    if (ohi < shi) s1 = team_lower_bound<G>(Sj, s0, s1, ohi + 1, lane);
     q0 = s0;
     q1 = s1;
 }
