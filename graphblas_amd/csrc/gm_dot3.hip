@@ -249,7 +249,11 @@ __device__ __forceinline__ int64_t bm_find(const RowBits &bm, const Pair &P, int
 
 // probe elements q0, q0+step, ... < q1 of the short row in the long row's
 // index; the iterations are independent, so several probes are in flight
-template <class S>
+// STOP: the lane leaves at its first element past the long row's range (the
+// short row is sorted), which spares the team path a second search; the
+// serial path keeps the loop free of that data-dependent exit so that its
+// unrolled loads issue together
+template <class S, bool STOP>
 __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P, int64_t q0, int64_t q1,
                                                int64_t step, bool &e, typename S::Z &z)
 {
@@ -264,9 +268,7 @@ __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P,
     for (int64_t q = q0; q < q1; q += step)
     {
         const int64_t x = Sj[q];
-        // the short row is sorted: past the long row's last column, so is
-        // every later element of this lane
-        if (x > P.hi) break;
+        if (STOP && x > P.hi) break;
         const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
         if (pos >= 0)
         {
@@ -353,8 +355,8 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
     if (P.empty) return false;
     if (P.probe)
     {
-        if (P.probe == 2) d3_probe_range<S>(g, P, P.pa, P.pa_end, 1, e, z);
-        else d3_probe_range<S>(g, P, P.pb, P.pb_end, 1, e, z);
+        if (P.probe == 2) d3_probe_range<S, false>(g, P, P.pa, P.pa_end, 1, e, z);
+        else d3_probe_range<S, false>(g, P, P.pb, P.pb_end, 1, e, z);
         if constexpr (S::plus_pair_wide) e = (z != 0);
         out = z;
         return e;
@@ -413,12 +415,14 @@ __device__ void dot_lane(const Dot3Args &g, const Pair &P, int lane, bool &e, ty
     const int64_t a = pa_end - pa, b = pb_end - pb;
     if (P.probe)
     {
-        // the short row, trimmed to the overlap of both ranges, interleaved
-        // over the lanes (coalesced reads), each probed in the long row's index
+        // the short row from its first element in the long row's range
+        // (teams: lanes stop past the range; a workgroup trims both ends),
+        // interleaved over the lanes (coalesced reads), each element probed
+        // in the long row's index
         int64_t q0, q1;
         if constexpr (G <= 64) d3_trim_short_team<G>(g, P, lane, q0, q1);
         else d3_trim_short(g, P, q0, q1);
-        d3_probe_range<S>(g, P, q0 + lane, q1, G, e, z);
+        d3_probe_range<S, (G <= 64)>(g, P, q0 + lane, q1, G, e, z);
         return;
     }
     const int64_t s = a < b ? a : b, l = a < b ? b : a;
