@@ -866,14 +866,14 @@ constexpr int LC_GMIN = 8;
 constexpr int LC_UNROLL = 8;
 constexpr int LC_NT = 1024;                   // 16 wavefronts share a chunk (one workgroup per CU)
 
-template <class S>
+template <class S, int G>
 __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *list,
                                                     const uint64_t *keys, int64_t count,
                                                     unsigned long long *ctr)
 {
     using Z = typename S::Z;
     static_assert(S::pattern, "LDS probing covers the pattern semirings");
-    constexpr int NW = LC_NT / 64;
+    constexpr int NW = LC_NT / G;            // teams of G lanes, one entry each
     __shared__ uint64_t s_bits[LC_WORDS];
     __shared__ uint64_t s_key[LC_SLICE];
     __shared__ int64_t s_cur[LC_SLICE], s_end[LC_SLICE], s_nx[LC_SLICE];
@@ -882,7 +882,9 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
     __shared__ uint8_t s_st[LC_SLICE];        // bit 0: result exists, 1: LDS entry, 2: visited
     __shared__ int64_t s_slice, s_lcur, s_lnext;
     __shared__ int s_ge;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & (G - 1), wv = threadIdx.x / G;
+    const int tbase = (int) (threadIdx.x & 63) & ~(G - 1);
+    const unsigned long long tmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
     const int64_t nslices = (count + LC_SLICE - 1) / LC_SLICE;
     const int x0 = (int) (blockIdx.x & 7);
     for (int k = 0; k < 8; k++)
@@ -947,8 +949,8 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                     (void) d3_pair(g, i, g.Mj[p], P);
                     bool e;
                     Z z;
-                    dot_lane<S, 64>(g, P, lane, e, z);
-                    team_reduce<S, 64>(e, z);
+                    dot_lane<S, G>(g, P, lane, e, z);
+                    team_reduce<S, G>(e, z);
                     if (lane == 0) d3_store<S>(g, p, e, z);
                 }
                 if (use_lds)
@@ -991,11 +993,11 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                             const int64_t end = s_end[t];
                             // first visit: skip the short row's elements below the long row
                             if (!(st & 4) && s_nx[t] < (int64_t) d.lo)
-                                cur = team_lower_bound<64>(Sj, cur, end, (int64_t) d.lo, lane);
+                                cur = team_lower_bound<G>(Sj, cur, end, (int64_t) d.lo, lane);
                             bool e = false;
                             Z z = S::identity();
                             int64_t nx = INT64_MAX;
-                            // LC_UNROLL windows of 64 elements per step, their
+                            // LC_UNROLL windows of G elements per step, their
                             // loads issued together (elements past the chunk are
                             // read again on the next visit, mostly from cache)
                             while (cur < end)
@@ -1004,7 +1006,7 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                                 #pragma unroll
                                 for (int u = 0; u < LC_UNROLL; u++)
                                 {
-                                    const int64_t q = cur + u * 64 + lane;
+                                    const int64_t q = cur + u * G + lane;
                                     xs[u] = q < end ? Sj[q] : INT64_MAX;
                                 }
                                 int taken = 0;
@@ -1025,13 +1027,13 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                                             else z = S::upd(z, tz);
                                         }
                                     }
-                                    const int nin = __popcll(__ballot(in));
+                                    const int nin = __popcll((__ballot(in) >> tbase) & tmask);
                                     if (!stop)
                                     {
                                         taken += nin;
-                                        if (nin < 64)
+                                        if (nin < G)
                                         {
-                                            nx = __shfl(x, nin, 64);
+                                            nx = __shfl(x, nin, G);
                                             stop = true;
                                         }
                                     }
@@ -1039,7 +1041,7 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                                 cur += taken;
                                 if (stop) break;
                             }
-                            team_reduce<S, 64>(e, z);
+                            team_reduce<S, G>(e, z);
                             if (lane == 0)
                             {
                                 bool ee = (st & 1) != 0;
@@ -1748,7 +1750,7 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
             if constexpr (S::pattern)
             {
                 ProfScope ps(b == 1 ? "dot3_wave_lds" : "dot3_block_lds");
-                k_dot3_lds<S><<<(unsigned) (cu_count() * 1), LC_NT, 0, st>>>(
+                k_dot3_lds<S, 64><<<(unsigned) (cu_count() * 1), LC_NT, 0, st>>>(
                     g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b],
                     ctr.as<unsigned long long>() + (b == 1 ? 24 : 32));
                 GM_HIP(hipGetLastError(), "k_dot3_lds");
