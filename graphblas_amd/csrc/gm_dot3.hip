@@ -249,10 +249,11 @@ __device__ __forceinline__ int64_t bm_find(const RowBits &bm, const Pair &P, int
 
 // probe elements q0, q0+step, ... < q1 of the short row in the long row's
 // index; the iterations are independent, so several probes are in flight
-// STOP: the lane leaves at its first element past the long row's range (the
-// short row is sorted), which spares the team path a second search; the
-// serial path keeps the loop free of that data-dependent exit so that its
-// unrolled loads issue together
+// STOP: the lane leaves after the first step that passes the long row's
+// range (the short row is sorted), which spares the team path a second
+// search; steps of four elements keep their loads and probes issued
+// together (team16 bin at RMAT 22: 25.9 ms with an exit per element, 23.1
+// ms in steps of four).  Without STOP the loop has no data-dependent exit.
 template <class S, bool STOP>
 __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P, int64_t q0, int64_t q1,
                                                int64_t step, bool &e, typename S::Z &z)
@@ -264,11 +265,46 @@ __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P,
     const int64_t *Sj = shortA ? g.Aj : g.Bj;
     const RowBits &bm = shortA ? g.bB : g.bA;
     const int64_t lp0 = shortA ? P.pb : P.pa;
+    if constexpr (STOP)
+    {
+        // four elements per step: their loads, then their probes, issue
+        // together; the lane stops after a step that passed the range
+        for (int64_t q = q0; q < q1; q += 4 * step)
+        {
+            int64_t xs[4];
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                const int64_t qq = q + u * step;
+                xs[u] = qq < q1 ? Sj[qq] : INT64_MAX;
+            }
+            bool done = false;
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                const int64_t x = xs[u];
+                if (x > P.hi) { done = true; break; }     // also the end of the row
+                const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
+                if (pos >= 0)
+                {
+                    const int64_t qq = q + u * step;
+                    const int64_t qa = shortA ? qq : pos, qb = shortA ? pos : qq;
+                    Z t = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                    if (!e) { z = t; e = true; }
+                    else if constexpr (S::any_min_k) { if (t < z) z = t; }
+                    else z = S::upd(z, t);
+                    if constexpr (S::is_any && !S::any_min_k) { done = true; break; }
+                    if (S::terminal(z) && !S::any_min_k) { done = true; break; }
+                }
+            }
+            if (done) break;
+        }
+        return;
+    }
     #pragma unroll 4
     for (int64_t q = q0; q < q1; q += step)
     {
         const int64_t x = Sj[q];
-        if (STOP && x > P.hi) break;
         const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
         if (pos >= 0)
         {
