@@ -391,8 +391,8 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
     if (P.empty) return false;
     if (P.probe)
     {
-        if (P.probe == 2) d3_probe_range<S, false>(g, P, P.pa, P.pa_end, 1, e, z);
-        else d3_probe_range<S, false>(g, P, P.pb, P.pb_end, 1, e, z);
+        if (P.probe == 2) d3_probe_range<S, true>(g, P, P.pa, P.pa_end, 1, e, z);
+        else d3_probe_range<S, true>(g, P, P.pb, P.pb_end, 1, e, z);
         if constexpr (S::plus_pair_wide) e = (z != 0);
         out = z;
         return e;
@@ -1200,27 +1200,41 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
                     const int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
                     bool e = false;
                     Z z = S::identity();
-                    #pragma unroll 4
-                    for (int64_t q = s0 + lane; q < s1; q += G)
+                    // steps of four elements per lane, loads issued together;
+                    // the lane stops after a step past the long row's range
+                    for (int64_t q0s = s0 + lane; q0s < s1; q0s += 4 * G)
                     {
-                        const int64_t x = Sj[q];
-                        if (x < (int64_t) d.lo || x > (int64_t) d.hi) continue;
-                        const int64_t dd = x - d.lo;
-                        const uint64_t word = s_bits[dd >> 6];
-                        const uint64_t bit = 1ull << (dd & 63);
-                        if (!(word & bit)) continue;
-                        Z tz;
-                        if constexpr (S::pattern)
-                            tz = S::mul(Z(1), Z(1), x);
-                        else
+                        int64_t xs[4];
+                        #pragma unroll
+                        for (int u = 0; u < 4; u++)
                         {
-                            const int64_t pos = d.p0 + (int64_t) s_rank[dd >> 6] + __popcll(word & (bit - 1));
-                            const int64_t qa = shortA ? q : pos, qb = shortA ? pos : q;
-                            tz = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                            const int64_t qq = q0s + u * G;
+                            xs[u] = qq < s1 ? Sj[qq] : INT64_MAX;
                         }
-                        if (!e) { z = tz; e = true; }
-                        else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
-                        else z = S::upd(z, tz);
+                        #pragma unroll
+                        for (int u = 0; u < 4; u++)
+                        {
+                            const int64_t x = xs[u];
+                            if (x < (int64_t) d.lo || x > (int64_t) d.hi) continue;
+                            const int64_t dd = x - d.lo;
+                            const uint64_t word = s_bits[dd >> 6];
+                            const uint64_t bit = 1ull << (dd & 63);
+                            if (!(word & bit)) continue;
+                            Z tz;
+                            if constexpr (S::pattern)
+                                tz = S::mul(Z(1), Z(1), x);
+                            else
+                            {
+                                const int64_t q = q0s + u * G;
+                                const int64_t pos = d.p0 + (int64_t) s_rank[dd >> 6] + __popcll(word & (bit - 1));
+                                const int64_t qa = shortA ? q : pos, qb = shortA ? pos : q;
+                                tz = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                            }
+                            if (!e) { z = tz; e = true; }
+                            else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                            else z = S::upd(z, tz);
+                        }
+                        if (xs[3] > (int64_t) d.hi) break;
                     }
                     team_reduce<S, G>(e, z);
                     if (lane == 0) d3_store<S>(g, p, e, z);
