@@ -1489,6 +1489,10 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_hist(const int64_t *__restrict_
                                                       const int64_t *__restrict__ Xj, int64_t nrows,
                                                       int64_t minlen, unsigned long long *hist)
 {
+    // per-workgroup histogram in LDS, then one atomic per nonzero bucket
+    __shared__ unsigned long long s_h[64];
+    if (threadIdx.x < 64) s_h[threadIdx.x] = 0;
+    __syncthreads();
     for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
          r += (int64_t) gridDim.x * D3_NT)
     {
@@ -1496,8 +1500,10 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_hist(const int64_t *__restrict_
         if (len < minlen) continue;
         const int64_t lo = Xj[p0], hi = Xj[p0 + len - 1];
         const int b = 63 - __clzll((unsigned long long) len);
-        atomicAdd(&hist[b], (unsigned long long) px_units(len, lo, hi));
+        atomicAdd(&s_h[b], (unsigned long long) px_units(len, lo, hi));
     }
+    __syncthreads();
+    if (threadIdx.x < 64 && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
 }
 
 // word offsets of the eligible rows (atomic bump) and the list of them
@@ -1507,14 +1513,45 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_assign(const int64_t *__restric
                                                         unsigned long long *wctr,
                                                         unsigned long long *lctr, uint32_t *list)
 {
-    for (int64_t r = (int64_t) blockIdx.x * D3_NT + threadIdx.x; r < nrows;
-         r += (int64_t) gridDim.x * D3_NT)
+    // one reservation per wavefront for its rows' units and list slots;
+    // every lane of a wavefront runs the same iterations
+    const int lane = threadIdx.x & 63;
+    for (int64_t r0 = (int64_t) blockIdx.x * D3_NT; r0 < nrows; r0 += (int64_t) gridDim.x * D3_NT)
     {
-        const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
-        if (len < T) continue;
-        const int64_t lo = Xj[p0], hi = Xj[p0 + len - 1];
-        off[r] = (int64_t) atomicAdd(wctr, (unsigned long long) px_units(len, lo, hi));
-        list[atomicAdd(lctr, 1ull)] = (uint32_t) r;
+        const int64_t r = r0 + threadIdx.x;
+        bool take = false;
+        unsigned long long u = 0;
+        if (r < nrows)
+        {
+            const int64_t p0 = Xp[r], len = Xp[r + 1] - p0;
+            if (len >= T)
+            {
+                take = true;
+                u = (unsigned long long) px_units(len, Xj[p0], Xj[p0 + len - 1]);
+            }
+        }
+        unsigned long long inc = u;
+        #pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const unsigned long long t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        const unsigned long long total = __shfl(inc, 63, 64);
+        const unsigned long long mask = __ballot(take);
+        unsigned long long wbase = 0, lbase = 0;
+        if (lane == 0 && mask != 0)
+        {
+            wbase = atomicAdd(wctr, total);
+            lbase = atomicAdd(lctr, (unsigned long long) __popcll(mask));
+        }
+        wbase = __shfl(wbase, 0, 64);
+        lbase = __shfl(lbase, 0, 64);
+        if (take)
+        {
+            off[r] = (int64_t) (wbase + inc - u);
+            list[lbase + (unsigned long long) __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t) r;
+        }
     }
 }
 
