@@ -51,9 +51,8 @@ __global__ void k_bt_set_base(BatchPart *parts, int64_t nparts, const int64_t *e
 
 // part owning stacked row / entry t: last r with base[r] <= t
 template <int64_t BatchPart::*FIELD>
-__device__ __forceinline__ int64_t bt_find(const BatchPart *parts, int64_t nparts, int64_t t)
+__device__ __forceinline__ int64_t bt_find_in(const BatchPart *parts, int64_t lo, int64_t hi, int64_t t)
 {
-    int64_t lo = 0, hi = nparts - 1;
     while (lo < hi)
     {
         const int64_t mid = (lo + hi + 1) >> 1;
@@ -62,38 +61,67 @@ __device__ __forceinline__ int64_t bt_find(const BatchPart *parts, int64_t npart
     return lo;
 }
 
+template <int64_t BatchPart::*FIELD>
+__device__ __forceinline__ int64_t bt_find(const BatchPart *parts, int64_t nparts, int64_t t)
+{
+    return bt_find_in<FIELD>(parts, 0, nparts - 1, t);
+}
+
+// A workgroup takes BT_TILE consecutive rows / entries: the parts owning
+// the tile's first and last element are searched once, and each element
+// searches only between them (usually one part)
+constexpr int BT_TILE = 256 * 16;
+
 // stacked row pointers
 __global__ void k_bt_rows(const BatchPart *parts, int64_t nparts, int64_t total_rows,
                           int64_t total_nnz, int64_t *P)
 {
-    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t <= total_rows;
-         t += (int64_t) gridDim.x * blockDim.x)
+    __shared__ int64_t s_r[2];
+    for (int64_t t0 = (int64_t) blockIdx.x * BT_TILE; t0 < total_rows; t0 += (int64_t) gridDim.x * BT_TILE)
     {
-        if (t == total_rows) { P[t] = total_nnz; continue; }
-        const int64_t r = bt_find<&BatchPart::row_base>(parts, nparts, t);
-        const BatchPart &b = parts[r];
-        P[t] = b.ent_base + (b.p[t - b.row_base] - b.p[0]);
+        const int64_t t1 = (t0 + BT_TILE < total_rows) ? t0 + BT_TILE : total_rows;
+        if (threadIdx.x < 2)
+            s_r[threadIdx.x] = bt_find<&BatchPart::row_base>(parts, nparts, threadIdx.x == 0 ? t0 : t1 - 1);
+        __syncthreads();
+        const int64_t r0 = s_r[0], r1 = s_r[1];
+        __syncthreads();
+        for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+        {
+            const int64_t r = bt_find_in<&BatchPart::row_base>(parts, r0, r1, t);
+            const BatchPart &b = parts[r];
+            P[t] = b.ent_base + (b.p[t - b.row_base] - b.p[0]);
+        }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) P[total_rows] = total_nnz;
 }
 
 // stacked column indices (+ values of size xs, iso parts expanded)
 __global__ void k_bt_ents(const BatchPart *parts, int64_t nparts, int64_t total_nnz, int64_t *J,
                           void *X, int xs)
 {
-    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < total_nnz;
-         t += (int64_t) gridDim.x * blockDim.x)
+    __shared__ int64_t s_r[2];
+    for (int64_t t0 = (int64_t) blockIdx.x * BT_TILE; t0 < total_nnz; t0 += (int64_t) gridDim.x * BT_TILE)
     {
-        const int64_t r = bt_find<&BatchPart::ent_base>(parts, nparts, t);
-        const BatchPart &b = parts[r];
-        const int64_t q = b.p[0] + (t - b.ent_base);
-        J[t] = b.j[q] + b.col_add;
-        if (X)
+        const int64_t t1 = (t0 + BT_TILE < total_nnz) ? t0 + BT_TILE : total_nnz;
+        if (threadIdx.x < 2)
+            s_r[threadIdx.x] = bt_find<&BatchPart::ent_base>(parts, nparts, threadIdx.x == 0 ? t0 : t1 - 1);
+        __syncthreads();
+        const int64_t r0 = s_r[0], r1 = s_r[1];
+        __syncthreads();
+        for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x)
         {
-            const int64_t qs = b.iso ? 0 : q;
-            if (xs == 8) ((uint64_t *) X)[t] = ((const uint64_t *) b.x)[qs];
-            else if (xs == 4) ((uint32_t *) X)[t] = ((const uint32_t *) b.x)[qs];
-            else if (xs == 2) ((uint16_t *) X)[t] = ((const uint16_t *) b.x)[qs];
-            else ((uint8_t *) X)[t] = ((const uint8_t *) b.x)[qs];
+            const int64_t r = bt_find_in<&BatchPart::ent_base>(parts, r0, r1, t);
+            const BatchPart &b = parts[r];
+            const int64_t q = b.p[0] + (t - b.ent_base);
+            J[t] = b.j[q] + b.col_add;
+            if (X)
+            {
+                const int64_t qs = b.iso ? 0 : q;
+                if (xs == 8) ((uint64_t *) X)[t] = ((const uint64_t *) b.x)[qs];
+                else if (xs == 4) ((uint32_t *) X)[t] = ((const uint32_t *) b.x)[qs];
+                else if (xs == 2) ((uint16_t *) X)[t] = ((const uint16_t *) b.x)[qs];
+                else ((uint8_t *) X)[t] = ((const uint8_t *) b.x)[qs];
+            }
         }
     }
 }
@@ -128,11 +156,11 @@ static int bt_stack(const std::vector<BatchPart> &hparts, int64_t total_rows, in
     {
         ProfScope ps("batch_stack");
         const int64_t cap = (int64_t) cu_count() * 16;
-        int64_t g1 = (total_rows + 256) / 256;
+        int64_t g1 = (total_rows + BT_TILE - 1) / BT_TILE;
         if (g1 > cap) g1 = cap;
         k_bt_rows<<<(unsigned) (g1 > 0 ? g1 : 1), 256, 0, st>>>(dparts.as<BatchPart>(), np, total_rows,
                                                               total_nnz, P.as<int64_t>());
-        int64_t g2 = (total_nnz + 255) / 256;
+        int64_t g2 = (total_nnz + BT_TILE - 1) / BT_TILE;
         if (g2 > cap) g2 = cap;
         if (total_nnz > 0)
             k_bt_ents<<<(unsigned) g2, 256, 0, st>>>(dparts.as<BatchPart>(), np, total_nnz, J.as<int64_t>(),
