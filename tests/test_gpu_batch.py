@@ -95,3 +95,30 @@ def test_batch_saxpy3_unmasked_fp64(grb):
     for r, (_, A) in enumerate(reqs):
         ref = O.saxpy3("plus_times_fp64", A, B)
         assert_csr_equal(slice_stacked(C, rb[r], rb[r + 1]), ref, fp_tol=1e-12, what=f"req {r}")
+
+
+def test_batch_dot3_empty_and_tiny_parts(grb):
+    """Stacking over many tiny requests, requests with no entries at all and
+    one large request: a 4096-element stacking tile then spans dozens of
+    parts, empty parts share their base with the next one, and the large
+    request fills whole tiles by itself."""
+    print("[batch] dot3 empty and tiny parts", flush=True)
+    rng = np.random.default_rng(19)
+    reqs = []
+    for r in range(300):
+        nm, nb, k = int(rng.integers(1, 12)), int(rng.integers(1, 12)), int(rng.integers(1, 20))
+        dens = 0.0 if r % 3 == 1 else 0.4
+        A = rand_csr(nm, k, dens, rng)
+        B = rand_csr(nb, k, 0.0 if r % 5 == 2 else 0.4, rng)
+        M = rand_csr(nm, nb, 0.0 if r % 7 == 3 else 0.5, rng, "int64")
+        reqs.append((M, A, B))
+    A = rand_csr(3000, 2000, 0.01, rng)
+    B = rand_csr(2500, 2000, 0.01, rng)
+    M = rand_csr(3000, 2500, 0.005, rng, "int64")
+    reqs.insert(150, (M, A, B))
+    C, rb, cb, split = gm_batch_dot3("GxB_PLUS_PAIR_INT64", reqs)
+    name, _ = SR["GxB_PLUS_PAIR_INT64"]
+    for r, (M, A, B) in enumerate(reqs):
+        ref = O.dot3(name, M, A, B)
+        assert_csr_equal(slice_stacked(C, rb[r], rb[r + 1], cb[r]), ref, what=f"req {r}")
+        assert_csr_equal(split[r], ref, what=f"split {r}")
