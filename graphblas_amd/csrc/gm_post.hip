@@ -5,8 +5,9 @@
 //                   GB_transpose, Source/transpose/GB_transpose.c, used when
 //                   the orientation rules of GB_AxB_meta.c:683-694,718-723
 //                   leave an explicit transpose).  Entries are keyed
-//                   (column << 32 | row) and radix-sorted (rocPRIM) with
-//                   their positions, so the rows of A' come out sorted; row
+//                   (column << 32 | row) and stably radix-sorted (rocPRIM)
+//                   on the column bits with their positions: the entries
+//                   enter in row order, so the rows of A' come out sorted; row
 //                   pointers by binary search of each row's first key, then
 //                   a gather of the values.
 //   gm_accum_mask   C<M, replace> = accum (C, T): the reference's
@@ -39,22 +40,39 @@ __global__ void k_post_zero(int64_t *x, int64_t n)
 // transpose
 //------------------------------------------------------------------------------
 
-// key of entry p: (column << 32) | row; pos[p] = p
+// key of entry p: (column << 32) | row; pos[p] = p.  A workgroup takes
+// TR_TILE consecutive entries: the rows holding its first and last entry are
+// found once, and each entry searches only between them.
+constexpr int TR_TILE = 256 * 16;
+
+__device__ __forceinline__ int64_t tr_row_of(const int64_t *__restrict__ Ap, int64_t lo, int64_t hi, int64_t p)
+{
+    // last i in [lo, hi] with Ap[i] <= p
+    while (lo < hi)
+    {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (Ap[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ void k_tr_keys(const int64_t *__restrict__ Ap, const int64_t *__restrict__ Aj, int64_t m,
                           int64_t nnz, uint64_t *__restrict__ keys, int64_t *__restrict__ pos)
 {
-    const int64_t stride = (int64_t) gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += stride)
+    __shared__ int64_t s_r[2];
+    for (int64_t t0 = (int64_t) blockIdx.x * TR_TILE; t0 < nnz; t0 += (int64_t) gridDim.x * TR_TILE)
     {
-        // row of entry p: last i with Ap[i] <= p
-        int64_t lo = 0, hi = m - 1;
-        while (lo < hi)
+        const int64_t t1 = (t0 + TR_TILE < nnz) ? t0 + TR_TILE : nnz;
+        if (threadIdx.x < 2) s_r[threadIdx.x] = tr_row_of(Ap, 0, m - 1, threadIdx.x == 0 ? t0 : t1 - 1);
+        __syncthreads();
+        const int64_t r0 = s_r[0], r1 = s_r[1];
+        __syncthreads();
+        for (int64_t p = t0 + threadIdx.x; p < t1; p += blockDim.x)
         {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (Ap[mid] <= p) lo = mid; else hi = mid - 1;
+            const int64_t i = tr_row_of(Ap, r0, r1, p);
+            keys[p] = ((uint64_t) Aj[p] << 32) | (uint64_t) i;
+            pos[p] = p;
         }
-        keys[p] = ((uint64_t) Aj[p] << 32) | (uint64_t) lo;
-        pos[p] = p;
     }
 }
 
@@ -322,21 +340,24 @@ extern "C" int gm_transpose(gm_csr_out *T, const gm_csr *A)
         const unsigned grid = post_grid(nnz, 256, (int64_t) cu_count() * 16);
         {
             ProfScope ps("transpose_keys");
-            k_tr_keys<<<grid, 256, 0, st>>>(A->p, A->j, m, nnz, k0.as<uint64_t>(), p0.as<int64_t>());
+            k_tr_keys<<<post_grid(nnz, TR_TILE, (int64_t) cu_count() * 16), 256, 0, st>>>(
+                A->p, A->j, m, nnz, k0.as<uint64_t>(), p0.as<int64_t>());
             GM_HIP(hipGetLastError(), "k_tr_keys");
         }
+        // the keys enter in row order and the radix sort is stable, so
+        // sorting by the column bits alone leaves each column's rows ascending
         int cbits = 1;
         while (cbits < 32 && ((int64_t) 1 << cbits) < n) cbits++;
-        const unsigned end_bit = 32u + (unsigned) cbits;
+        const unsigned begin_bit = 32u, end_bit = 32u + (unsigned) cbits;
         size_t tb = 0;
         GM_HIP(rocprim::radix_sort_pairs(nullptr, tb, k0.as<uint64_t>(), k1.as<uint64_t>(),
-                                         p0.as<int64_t>(), p1.as<int64_t>(), (size_t) nnz, 0u, end_bit,
+                                         p0.as<int64_t>(), p1.as<int64_t>(), (size_t) nnz, begin_bit, end_bit,
                                          st), "transpose sort size");
         if (!tmp.alloc(tb)) return set_error(GM_OUT_OF_MEMORY, "gm_transpose: sort storage");
         {
             ProfScope ps("transpose_sort");
             GM_HIP(rocprim::radix_sort_pairs(tmp.ptr, tb, k0.as<uint64_t>(), k1.as<uint64_t>(),
-                                             p0.as<int64_t>(), p1.as<int64_t>(), (size_t) nnz, 0u,
+                                             p0.as<int64_t>(), p1.as<int64_t>(), (size_t) nnz, begin_bit,
                                              end_bit, st), "transpose sort");
         }
         {
