@@ -61,6 +61,29 @@ def test_transpose(grb, shape, tname):
     assert_csr_equal(got, ref, what="transpose")
 
 
+def test_transpose_large(grb):
+    """About 2.2M entries over 2^19 columns, two of them held by a third of
+    the rows: the sort runs several radix passes over the column bits, and
+    every column's rows must still come out ascending."""
+    print("[post] transpose large", flush=True)
+    rng = np.random.default_rng(77)
+    m, n = 300000, 1 << 19
+    keys = rng.integers(0, m * n, 2_000_000, dtype=np.int64)
+    hub_rows = np.arange(0, m, 3, dtype=np.int64)
+    keys = np.unique(np.concatenate([keys, hub_rows * n + 7, hub_rows * n + (n - 1)]))
+    rows, cols = keys // n, keys % n
+    p = np.zeros(m + 1, dtype=np.int64)
+    np.add.at(p, rows + 1, 1)
+    A = W.Csr(m, n, np.cumsum(p), cols, rng.standard_normal(len(keys)))
+    dA = DevCsr(A, TYPES["fp64"][0])
+    out = capi.GmCsrOut()
+    rc = capi.lib().gm_transpose(ctypes.byref(out), ctypes.byref(dA.view))
+    dA.free()
+    assert rc == 0, capi.last_error()
+    got = download(out, np.float64)
+    assert_csr_equal(got, O.transpose(A), what="transpose large")
+
+
 @pytest.mark.parametrize("op", list(OPS))
 @pytest.mark.parametrize("tname", ["int32", "int64", "fp64", "bool"])
 def test_accum_mask_ops(grb, op, tname):
