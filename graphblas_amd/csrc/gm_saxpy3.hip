@@ -174,7 +174,9 @@ __device__ __forceinline__ T team_sum(T v)
 }
 
 //------------------------------------------------------------------------------
-// max row lengths of A and B (decides whether every row is "small")
+// max row lengths of A and B (decides whether every row is "small"):
+// out[0] / out[1]; and nnz(A) / nnz(B) in out[2] / out[5], so that one copy
+// brings all four to the host
 //------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__ Ap, int64_t m,
@@ -184,6 +186,11 @@ __global__ __launch_bounds__(SX_NT) void k_sx_maxlen(const int64_t *__restrict__
     __shared__ unsigned long long s[2][SX_NT / 64];
     unsigned long long ma = 0, mb = 0;
     const int64_t lim = m > kb ? m : kb;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        out[2] = (unsigned long long) Ap[m];
+        out[5] = (unsigned long long) Bp[kb];
+    }
     // four consecutive rows per thread and pass, so that one pass of a
     // modest grid has all of its loads in flight together
     for (int64_t i0 = ((int64_t) blockIdx.x * SX_NT + threadIdx.x) * 4; i0 < lim;
@@ -1200,11 +1207,9 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
                 A->p, m, B->p, B->nrows, g.cnt + 8);
             GM_HIP(hipGetLastError(), "k_sx_maxlen");
         }
-        GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+        // counters 8..13: max |A(i,:)|, max |B(k,:)|, nnz(A), -, -, nnz(B)
+        GM_HIP(hipMemcpyAsync(hp + 8, g.cnt + 8, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
                "maxlen d2h");
-        GM_HIP(hipMemcpyAsync(hp + 10, A->p + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
-        GM_HIP(hipMemcpyAsync(hp + 13, B->p + B->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st),
-               "nnz(B)");
         GM_HIP(hipStreamSynchronize(st), "maxlen sync");
         maxA = (int64_t) hp[8];
         maxB = (int64_t) hp[9];
