@@ -1400,37 +1400,13 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_compact(const int64_t *__restric
     const int64_t *__restrict__ tile_off, int64_t *__restrict__ Cp,
     int64_t *__restrict__ Ci, Z *__restrict__ Cx)
 {
-    __shared__ int64_t lds[D3_NT / 64 + 1];
+    constexpr int NW = D3_NT / 64;
+    static_assert(D3_CT_IPT * NW <= 64, "one wavefront scans the per-step counts");
     __shared__ int32_t s_pre[D3_CT_TILE + 1];
+    __shared__ int32_t s_cnt[D3_CT_IPT * NW];
     __shared__ int64_t s_row0;
     const int64_t t0 = (int64_t) blockIdx.x * D3_CT_TILE;
-    const int64_t base = t0 + (int64_t) threadIdx.x * D3_CT_IPT;
-    uint8_t f[D3_CT_IPT];
-    int64_t c = 0;
-    #pragma unroll
-    for (int t = 0; t < D3_CT_IPT; t++)
-    {
-        const int64_t p = base + t;
-        f[t] = (p < mnz) ? live[p] : 0;
-        c += f[t];
-    }
-    int64_t tot;
-    const int64_t ex = block_exclusive_scan<int64_t, D3_NT>(c, lds, &tot);
-    const int64_t off = tile_off[blockIdx.x];
-    int64_t run = ex;
-    #pragma unroll
-    for (int t = 0; t < D3_CT_IPT; t++)
-    {
-        const int64_t p = base + t;
-        s_pre[threadIdx.x * D3_CT_IPT + t] = (int32_t) run;
-        if (f[t])
-        {
-            Ci[off + run] = Mj[p];
-            if constexpr (!ISO) Cx[off + run] = W[p];
-        }
-        run += f[t];
-    }
-    if (threadIdx.x == D3_NT - 1) s_pre[D3_CT_TILE] = (int32_t) run;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (threadIdx.x == 0)
     {
         // first row whose start lies at or after t0
@@ -1441,6 +1417,46 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_compact(const int64_t *__restric
             if (Mp[mid] < t0) lo = mid + 1; else hi = mid;
         }
         s_row0 = lo;
+    }
+    // step t covers entries t0 + t*D3_NT + [0, D3_NT): a wavefront's lanes
+    // read and write consecutive entries; the live ones of a step are placed
+    // by a ballot, the steps and wavefronts in entry order by one scan
+    unsigned long long bal[D3_CT_IPT];
+    #pragma unroll
+    for (int t = 0; t < D3_CT_IPT; t++)
+    {
+        const int64_t p = t0 + (int64_t) t * D3_NT + threadIdx.x;
+        bal[t] = __ballot((p < mnz) && live[p] != 0);
+        if (lane == 0) s_cnt[t * NW + wid] = __popcll(bal[t]);
+    }
+    __syncthreads();
+    if (wid == 0)
+    {
+        const int v = lane < D3_CT_IPT * NW ? s_cnt[lane] : 0;
+        int incl = v;
+        #pragma unroll
+        for (int d = 1; d < 64; d <<= 1)
+        {
+            const int o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane < D3_CT_IPT * NW) s_cnt[lane] = incl - v;
+        if (lane == 63) s_pre[D3_CT_TILE] = incl;
+    }
+    __syncthreads();
+    const int64_t off = tile_off[blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    #pragma unroll
+    for (int t = 0; t < D3_CT_IPT; t++)
+    {
+        const int64_t p = t0 + (int64_t) t * D3_NT + threadIdx.x;
+        const int pos = s_cnt[t * NW + wid] + __popcll(bal[t] & below);
+        s_pre[t * D3_NT + threadIdx.x] = pos;
+        if ((bal[t] >> lane) & 1ull)
+        {
+            Ci[off + pos] = Mj[p];
+            if constexpr (!ISO) Cx[off + pos] = W[p];
+        }
     }
     __syncthreads();
     // rows starting inside this tile; the last tile also takes rows that
