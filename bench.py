@@ -1,23 +1,28 @@
 """Benchmark of GrB_mxm on one MI355X.
 
 Default workload (one step = one GrB_mxm call through the C API, inputs
-already resident in HBM):
+already resident in HBM): the triangle count C<L> = L*U' with the plus_pair
+int64 semiring (GrB_DESC_ST1, the dot3 method) on an RMAT scale-22 graph
+(n = 4,194,304, edge factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, symmetrised,
+deduplicated, no self-loops, no degree presort).
 
-    C = A*B, plus_times fp64 (saxpy3), A and B n = 2^20 square CSR with
-    exactly 8 distinct uniform columns per row, values uniform [0,1).
+value = semiring multiplies per second over the timed steps (one multiply
+per common neighbour: sum(C), the triangle count).  The roofline object
+covers all dot3 kernels of a call together, timed with HIP events on the
+library stream over the timed region: achieved = algorithmic bytes of one
+call (probe model: the shorter list of every mask entry read once, M read
+once, C written once) / the kernels' summed duration.  The result of the
+first call is checked against the committed fixture
+tests/golden/fullsize_rmat22.json (nnz(C), sum(C)).  cpu_baseline times the
+CPU oracle (a plain-C OpenMP restatement of the reference's dot3) on a
+bounded sample of the same mask (its first n/64 rows, which hold the hub
+rows).
 
-value = semiring multiplies per second over the timed steps (2^26 mults per
-call, from a host pass over the inputs).  The roofline object covers the
-dominant kernel (the saxpy3 numeric kernel for 8-entry rows), timed with HIP
-events on the library stream over the timed region: achieved = algorithmic
-bytes of one launch / its average duration.  cpu_baseline times the CPU
-oracle (a plain-C OpenMP restatement of the reference's saxpy3) on the host.
-
-Other workloads (--workload): dot3-rmat14 / dot3-rmat22 (C<L> = L*U',
-plus_pair int64, the triangle count); saxpy3-tricount-rmat22 (the same count
-as C<L> = L*L with the GxB_AxB_SAXPY hint); mxm-tricount-rmat22 (C<L> = L*L
-with no hint: GrB_mxm's flop rule runs dot3 on L'); batched-4096;
-served-rmat20.
+Other workloads (--workload): saxpy3-er20 (C = A*B plus_times fp64, A and B
+n = 2^20 with 8 uniform columns per row); dot3-rmat14; saxpy3-tricount-rmat22
+(the same count as C<L> = L*L with the GxB_AxB_SAXPY hint);
+mxm-tricount-rmat22 (C<L> = L*L with no hint: GrB_mxm's flop rule runs dot3
+on L'); batched-4096; served-rmat20.
 """
 from __future__ import annotations
 
@@ -57,7 +62,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="saxpy3-er20",
+    ap.add_argument("--workload", default="dot3-rmat22",
                     choices=["saxpy3-er20", "dot3-rmat14", "dot3-rmat22", "served-rmat20",
                              "batched-4096", "saxpy3-tricount-rmat22", "mxm-tricount-rmat22"])
     ap.add_argument("--requests", type=int, default=4096,
@@ -114,6 +119,18 @@ def timed_steps(capi, fn, steps, warmup):
     capi.prof_enable(False)
     kern = capi.prof_get()
     return (t1 - t0) * 1e3 / steps, kern
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's 'Model name', from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(fn, budget_s, unit_per_call):
@@ -235,6 +252,7 @@ def bench_served(args, capi, W):
             O.saxpy3("any_secondi_int64", F, G, M=F, mask_comp=True)
         v, cores, reps, el = cpu_baseline(cpu_fn, args.cpu_seconds, unit)
         out["cpu_baseline"] = {"value": v, "unit": "mults/s", "cores": cores, "kind": "port",
+                               "cpu_model": cpu_model(),
                                "sample": f"first {nr} requests stacked into one C<!F>=F*A: "
                                          f"{reps} calls in {el:.1f} s"}
     else:
@@ -323,25 +341,32 @@ def bench_batched(args, capi, W):
         mults += r.value
         for ptr in (out.p, out.j, out.x):
             L.gm_free(ptr)
+    # both legs timed the same way, without the per-kernel events; the
+    # kernel split comes from a separate profiled pass of each leg
     L.gm_synchronize()
-    capi.prof_reset()
-    capi.prof_enable(True)
     t0 = time.perf_counter()
     plain(False)
     L.gm_synchronize()
     t_plain = time.perf_counter() - t0
-    capi.prof_enable(False)
-    plain_kernel_ms = sum(v[1] for v in capi.prof_get().values())
     steps = max(1, args.steps)
-    capi.prof_reset()
-    capi.prof_enable(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         batched(False)
     L.gm_synchronize()
     t_b = (time.perf_counter() - t0) / steps
+    capi.prof_reset()
+    capi.prof_enable(True)
+    plain(False)
+    L.gm_synchronize()
+    capi.prof_enable(False)
+    plain_kernel_ms = sum(v[1] for v in capi.prof_get().values())
+    capi.prof_reset()
+    capi.prof_enable(True)
+    batched(False)
+    L.gm_synchronize()
     capi.prof_enable(False)
     kern = capi.prof_get()
+    steps_prof = 1
     nnz = int(db.nzbase[-1])
     ns = np.array([s.n for s in specs])
     out = {"metric": "batched uneven mxm: semiring mults/s, batched vs plain one-call-per-request",
@@ -362,7 +387,7 @@ def bench_batched(args, capi, W):
            "results_identical_batched_vs_plain": bool(same),
            "generate_ms": t_gen * 1e3,
            "tuning": os.path.relpath(tuning, ROOT) if tuning else None,
-           "kernels_ms_per_step": {k: v[1] / steps for k, v in sorted(kern.items())},
+           "kernels_ms_per_step": {k: v[1] / steps_prof for k, v in sorted(kern.items())},
            "roofline": None}
     if not args.no_cpu:
         import pyoracle as O
@@ -381,6 +406,7 @@ def bench_batched(args, capi, W):
                 O.dot3(name, A, A, A, mask_struct=False)
         v, cores, reps, el = cpu_baseline(cpu_fn, args.cpu_seconds, units)
         out["cpu_baseline"] = {"value": v, "unit": "mults/s", "cores": cores, "kind": "port",
+                               "cpu_model": cpu_model(),
                                "sample": f"the first {len(hosts)} requests with n <= 4096, one "
                                          f"oracle dot3 each: {reps} passes in {el:.1f} s"}
     else:
@@ -530,6 +556,16 @@ def main():
         ntri = capi.reduce_sum_int64(C)
         nnzC = C.nvals
         mults = int(ntri)      # plus_pair: every multiply is one common neighbour
+        if scale == 22:
+            # the first call's result against the oracle's committed fixture
+            with open(os.path.join(ROOT, "tests", "golden", "fullsize_rmat22.json")) as f:
+                gold = json.load(f)
+            ok = (int(ntri) == gold["sum_C"] and int(nnzC) == gold["nnz_C"])
+            out["check"] = {"against": "tests/golden/fullsize_rmat22.json (oracle's nnz(C), sum(C))",
+                            "ok": bool(ok)}
+            if not ok:
+                raise SystemExit(f"dot3-rmat22 result differs from the fixture: sum {ntri} "
+                                 f"(want {gold['sum_C']}), nnz {nnzC} (want {gold['nnz_C']})")
         balg_merge, balg = dot3_alg_bytes(L, L, U, nnzC)
         ms, kern = timed_steps(capi, step, args.steps, args.warmup)
         # dot3 spreads one call over several kernels: the roofline covers
@@ -581,6 +617,7 @@ def main():
     if not args.no_cpu:
         v, cores, reps, el = cpu_baseline(cpu_fn, args.cpu_seconds, cpu_unit)
         out["cpu_baseline"] = {"value": v, "unit": "mults/s", "cores": cores, "kind": "port",
+                               "cpu_model": cpu_model(),
                                "sample": f"{cpu_sample}: {reps} calls in {el:.1f} s"}
     else:
         out["cpu_baseline"] = None

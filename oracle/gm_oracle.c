@@ -299,3 +299,66 @@ double oracle_neumaier_sum(const double *x, int64_t n)
     }
     return s + c;
 }
+
+// C = A*B under plus_times fp64 with every entry accumulated in double-double
+// arithmetic: each product a*b is split exactly into p + e (e = fma(a, b, -p),
+// the error-free product) and both parts are added to the entry's
+// (hi, lo) pair with Knuth's two-sum, so the only rounding left is that of
+// the final hi + lo.  This is the compensated FP64 reference against which
+// the summation error of both the sequential path (the reference's
+// ascending-k order, GB_AxB_saxpy3_coarseHash_phase5.c:22-93) and the GPU
+// path is stated.  The pattern (Cp, Ci, sorted rows) comes from
+// oracle_saxpy3; Cx receives the compensated values in the same positions.
+// Returns 0, or -1 when a column of A*B is missing from Ci (caller error).
+int64_t oracle_saxpy3_plus_times_fp64_dd(const or_csr *A, const or_csr *B, const int64_t *Cp,
+                                         const int64_t *Ci, double *Cx)
+{
+    const int64_t m = A->nrows, n = B->ncols;
+    const double *Ax = (const double *) A->x, *Bx = (const double *) B->x;
+    int64_t bad = 0;
+    #pragma omp parallel reduction(+:bad)
+    {
+        int64_t *pos = (int64_t *) malloc(sizeof(int64_t) * (size_t) (n > 0 ? n : 1));
+        double *hi = NULL, *lo = NULL;
+        int64_t cap = 0;
+        if (pos) for (int64_t t = 0; t < n; t++) pos[t] = -1;
+        #pragma omp for schedule(dynamic, 256)
+        for (int64_t i = 0; i < m; i++)
+        {
+            if (!pos) { bad++; continue; }
+            const int64_t c0 = Cp[i], c1 = Cp[i + 1], len = c1 - c0;
+            if (len > cap)
+            {
+                free(hi); free(lo);
+                cap = 2 * len;
+                hi = (double *) malloc(sizeof(double) * (size_t) cap);
+                lo = (double *) malloc(sizeof(double) * (size_t) cap);
+            }
+            for (int64_t q = 0; q < len; q++) { pos[Ci[c0 + q]] = q; hi[q] = 0.0; lo[q] = 0.0; }
+            for (int64_t pa = A->p[i]; pa < A->p[i + 1]; pa++)
+            {
+                const int64_t k = A->j[pa];
+                const double a = Ax[A->iso ? 0 : pa];
+                for (int64_t pb = B->p[k]; pb < B->p[k + 1]; pb++)
+                {
+                    const int64_t q = pos[B->j[pb]];
+                    if (q < 0) { bad++; continue; }
+                    const double b = Bx[B->iso ? 0 : pb];
+                    const double p = a * b, e = fma(a, b, -p);
+                    // (hi, lo) += p: two-sum, then fold the error and e into lo
+                    const double s = hi[q] + p, bb = s - hi[q];
+                    const double err = (hi[q] - (s - bb)) + (p - bb);
+                    hi[q] = s;
+                    lo[q] += err + e;
+                }
+            }
+            for (int64_t q = 0; q < len; q++)
+            {
+                Cx[c0 + q] = hi[q] + lo[q];
+                pos[Ci[c0 + q]] = -1;
+            }
+        }
+        free(pos); free(hi); free(lo);
+    }
+    return bad ? -1 : 0;
+}

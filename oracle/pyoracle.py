@@ -63,6 +63,8 @@ def lib():
         L.oracle_sum_i64.argtypes = [P, ctypes.c_int64]
         L.oracle_neumaier_sum.restype = ctypes.c_double
         L.oracle_neumaier_sum.argtypes = [P, ctypes.c_int64]
+        L.oracle_saxpy3_plus_times_fp64_dd.restype = ctypes.c_int64
+        L.oracle_saxpy3_plus_times_fp64_dd.argtypes = [P, P, P, P, P]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -123,6 +125,20 @@ def saxpy3(semiring: str, A, B, M=None, mask_comp=False, mask_struct=True):
     lib().oracle_saxpy3(code, Mref, int(mask_comp), int(mask_struct), Aref, Bref,
                         _ptr(Cp), _ptr(Ci), _ptr(Cx), 1)
     return Csr(m, B.ncols, Cp, Ci[:tot].copy(), Cx[:tot].copy())
+
+
+def saxpy3_plus_times_fp64_dd(A, B, C):
+    """Compensated (double-double) values of C = A*B under plus_times fp64
+    on the pattern of C (sorted rows, e.g. from saxpy3())."""
+    keep = []
+    Cp = np.ascontiguousarray(C.p, dtype=np.int64)
+    Ci = np.ascontiguousarray(C.j, dtype=np.int64)
+    Cx = np.empty(max(len(Ci), 1), dtype=np.float64)
+    rc = lib().oracle_saxpy3_plus_times_fp64_dd(_csr(A, keep), _csr(B, keep), _ptr(Cp), _ptr(Ci),
+                                                 _ptr(Cx))
+    if rc != 0:
+        raise RuntimeError("oracle_saxpy3_plus_times_fp64_dd: pattern of C misses a product")
+    return Cx[:len(Ci)]
 
 
 def neumaier_sum(x: np.ndarray) -> float:

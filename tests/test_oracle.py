@@ -234,6 +234,39 @@ def test_saxpy3_plus_times_vs_scipy_and_fsum():
     assert worst <= 64 * np.finfo(np.float64).eps
 
 
+def test_compensated_saxpy3_is_correctly_rounded():
+    """The double-double oracle equals the exact sum of the exact products
+    (rational arithmetic), rounded to double, within half an ulp; a dropped
+    term or a product left in double would fail it on these ill-scaled rows."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    A = power_law_csr(80, 10, rng, values="fp64")
+    B = power_law_csr(80, 10, rng, values="fp64")
+    # wide exponent range and mixed signs, so plain double sums round
+    A.x = A.x * np.exp2(rng.integers(-30, 30, A.nnz)) * rng.choice([-1.0, 1.0], A.nnz)
+    B.x = B.x * np.exp2(rng.integers(-30, 30, B.nnz))
+    C = O.saxpy3("plus_times_fp64", A, B)
+    dd = O.saxpy3_plus_times_fp64_dd(A, B, C)
+    worst, plain_worst = 0.0, 0.0
+    for i in range(A.nrows):
+        acc = {}
+        for pa in range(A.p[i], A.p[i + 1]):
+            k = A.j[pa]
+            for pb in range(B.p[k], B.p[k + 1]):
+                jj = int(B.j[pb])
+                acc[jj] = acc.get(jj, Fraction(0)) + Fraction(A.x[pa]) * Fraction(B.x[pb])
+        for q in range(C.p[i], C.p[i + 1]):
+            exact = acc[int(C.j[q])]
+            if exact == 0:
+                assert dd[q] == 0.0
+                continue
+            worst = max(worst, float(abs(Fraction(dd[q]) - exact) / abs(exact)))
+            plain_worst = max(plain_worst, float(abs(Fraction(C.x[q]) - exact) / abs(exact)))
+    assert worst <= 2.0 ** -53 * (1 + 1e-9), worst
+    # the check is not vacuous: the sequential sum is visibly worse here
+    assert plain_worst > 4 * worst
+
+
 def test_saxpy3_ascending_k_order_bits():
     """The oracle's fp64 sum is the sequential ascending-k sum (no FMA)."""
     rng = np.random.default_rng(1)
