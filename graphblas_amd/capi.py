@@ -109,20 +109,27 @@ class GmSaxpy3Request(ctypes.Structure):
 
 class GmServerConfig(ctypes.Structure):
     _fields_ = [("workers", ctypes.c_int32), ("max_batch_reqs", ctypes.c_int32),
-                ("flop_budget", ctypes.c_int64), ("reserved", ctypes.c_int64)]
+                ("flop_budget", ctypes.c_int64), ("mem_budget", ctypes.c_int64)]
+
+
+class GmServerRequestOpts(ctypes.Structure):
+    _fields_ = [("max_steps", ctypes.c_int64), ("keep_visited", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class GmServerResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("pad", ctypes.c_int32), ("k", ctypes.c_int64),
                 ("nvals", ctypes.c_int64), ("flops", ctypes.c_int64),
                 ("latency_us", ctypes.c_double), ("queue_us", ctypes.c_double),
-                ("batch_id", ctypes.c_int64), ("batch_size", ctypes.c_int64)]
+                ("batch_id", ctypes.c_int64), ("batch_size", ctypes.c_int64),
+                ("steps", ctypes.c_int64), ("vnvals", ctypes.c_int64)]
 
 
 class GmServerCounters(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_int64), ("batches", ctypes.c_int64),
                 ("pending", ctypes.c_int64), ("max_in_flight", ctypes.c_int64),
-                ("mean_batch", ctypes.c_double), ("flops", ctypes.c_double)]
+                ("mean_batch", ctypes.c_double), ("flops", ctypes.c_double),
+                ("levels", ctypes.c_int64), ("oom_splits", ctypes.c_int64)]
 
 
 class GmProfRecord(ctypes.Structure):
@@ -205,12 +212,15 @@ def lib():
         "gm_free": ([P], None),
         "gm_arena_trim": ([], ctypes.c_int),
         "gm_arena_bytes_in_use": ([], I64),
+        "gm_arena_bytes_cached": ([], I64),
         "gm_debug_fail_after": ([I64], None),
         "gm_memcpy_h2d": ([P, P, ctypes.c_size_t], ctypes.c_int),
         "gm_memcpy_d2h": ([P, P, ctypes.c_size_t], ctypes.c_int),
         "gm_memcpy_d2d": ([P, P, ctypes.c_size_t], ctypes.c_int),
         "gm_memset": ([P, ctypes.c_int, ctypes.c_size_t], ctypes.c_int),
         "gm_synchronize": ([], ctypes.c_int),
+        "gm_set_stream": ([P], ctypes.c_int),
+        "gm_get_stream": ([], P),
         "gm_prof_enable": ([ctypes.c_int], None),
         "gm_prof_reset": ([], None),
         "gm_prof_get": ([ctypes.POINTER(GmProfRecord), ctypes.c_int], ctypes.c_int),
@@ -219,6 +229,12 @@ def lib():
         "gm_dot3": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.c_bool,
                      ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
                      ctypes.POINTER(GmDot3Params)], ctypes.c_int),
+        "gm_AxB_dot3_branch": ([ctypes.POINTER(GmCsr), ctypes.c_bool, ctypes.POINTER(GmCsr),
+                                ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring), ctypes.c_bool],
+                               ctypes.c_bool),
+        "gm_AxB_dot3": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmCsr), ctypes.c_bool,
+                         ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
+                         ctypes.c_bool], ctypes.c_int),
         "gm_saxpy3_branch": ([ctypes.POINTER(GmCsr), ctypes.c_bool, ctypes.c_bool,
                               ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                               ctypes.POINTER(GmSemiring)], ctypes.c_bool),
@@ -273,6 +289,9 @@ def lib():
         "gm_server_create": ([ctypes.POINTER(GmCsr), ctypes.POINTER(GmSemiring),
                               ctypes.POINTER(GmServerConfig)], P),
         "gm_server_submit": ([P, P, I64, P, P], I64),
+        "gm_server_submit_bfs": ([P, P, I64, P, P, ctypes.POINTER(GmServerRequestOpts)], I64),
+        "gm_server_fetch_visited": ([P, I64, P, P, P], ctypes.c_int),
+        "gm_server_step": ([P], I64),
         "gm_server_wait": ([P, I64, ctypes.POINTER(GmServerResult)], ctypes.c_int),
         "gm_server_fetch": ([P, I64, P, P, P], ctypes.c_int),
         "gm_server_release": ([P, I64], ctypes.c_int),
@@ -619,17 +638,19 @@ class Server:
     graph: a gm_csr view whose device arrays outlive the server."""
 
     def __init__(self, graph: GmCsr, semiring=(4, 4, 8), workers=2, max_batch_reqs=256,
-                 flop_budget=1 << 24):
+                 flop_budget=1 << 24, mem_budget=0):
         self._graph = graph
         self._sr = GmSemiring(*semiring)
         self._zt = semiring[2]
-        cfg = GmServerConfig(workers, max_batch_reqs, flop_budget, 0)
+        cfg = GmServerConfig(workers, max_batch_reqs, flop_budget, mem_budget)
         self.h = lib().gm_server_create(ctypes.byref(self._graph), ctypes.byref(self._sr),
                                         ctypes.byref(cfg))
         if not self.h:
             raise GrBError(-1, last_error())
 
-    def submit(self, sources, vis=None) -> int:
+    def submit(self, sources, vis=None, max_steps=1, keep_visited=False) -> int:
+        """Queue a BFS of max_steps levels (<= 0: until the frontier is empty)
+        from the sources; vis = (vis_p, vis_j) marks vertices visited."""
         src = np.ascontiguousarray(sources, dtype=np.int64)
         vp = vj = None
         if vis is not None:
@@ -637,9 +658,11 @@ class Server:
             vj = np.ascontiguousarray(vis[1], dtype=np.int64)
             if len(vj) == 0:
                 vj = np.zeros(1, dtype=np.int64)
-        rid = lib().gm_server_submit(self.h, src.ctypes.data, len(src),
-                                     vp.ctypes.data if vp is not None else None,
-                                     vj.ctypes.data if vj is not None else None)
+        opts = GmServerRequestOpts(max_steps, 1 if keep_visited else 0, 0)
+        rid = lib().gm_server_submit_bfs(self.h, src.ctypes.data, len(src),
+                                         vp.ctypes.data if vp is not None else None,
+                                         vj.ctypes.data if vj is not None else None,
+                                         ctypes.byref(opts))
         if rid <= 0:
             raise GrBError(int(rid), last_error())
         return rid
@@ -661,6 +684,25 @@ class Server:
         if rc != 0:
             raise GrBError(rc, last_error())
         return p, j[:res.nvals], x[:res.nvals]
+
+    def fetch_visited(self, rid, res: GmServerResult):
+        """(row pointers, column indices, parents) of a completed request's
+        visited set (submitted with keep_visited)."""
+        dt = {8: np.int64, 6: np.int32, 1: np.uint8, 11: np.float64}[self._zt]
+        p = np.empty(res.k + 1, dtype=np.int64)
+        j = np.empty(max(res.vnvals, 1), dtype=np.int64)
+        x = np.empty(max(res.vnvals, 1), dtype=dt)
+        rc = lib().gm_server_fetch_visited(self.h, rid, p.ctypes.data, j.ctypes.data, x.ctypes.data)
+        if rc != 0:
+            raise GrBError(rc, last_error())
+        return p, j[:res.vnvals], x[:res.vnvals]
+
+    def step(self) -> int:
+        """Without worker threads: run one batch here; its request count."""
+        n = lib().gm_server_step(self.h)
+        if n < 0:
+            raise GrBError(int(n), last_error())
+        return n
 
     def release(self, rid):
         lib().gm_server_release(self.h, rid)

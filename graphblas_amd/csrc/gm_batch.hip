@@ -22,18 +22,6 @@
 namespace gm
 {
 
-struct BatchPart
-{
-    const int64_t *p, *j;
-    const void *x;
-    int64_t nrows;
-    int64_t row_base;       // first stacked row
-    int64_t col_add;        // added to every column index
-    int64_t ent_base;       // first stacked entry (filled on the device)
-    int32_t iso;
-    int32_t pad;
-};
-
 // nnz of every part (its last row pointer)
 __global__ void k_bt_nnz(const BatchPart *parts, int64_t nparts, int64_t *nnz)
 {
@@ -128,7 +116,7 @@ __global__ void k_bt_ents(const BatchPart *parts, int64_t nparts, int64_t total_
 
 // Stacked copy of parts; arrays allocated here (released by the caller's
 // DevBufs).  Returns the stacked gm_csr view.
-static int bt_stack(const std::vector<BatchPart> &hparts, int64_t total_rows, int64_t ncols,
+int bt_stack(const std::vector<BatchPart> &hparts, int64_t total_rows, int64_t ncols,
                     bool values, int32_t type, DevBuf &dparts, DevBuf &P, DevBuf &J, DevBuf &X,
                     gm_csr *out)
 {

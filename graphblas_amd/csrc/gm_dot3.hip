@@ -2236,6 +2236,29 @@ extern "C" bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *
     return true;
 }
 
+// multipliers that give the same value with their operands swapped
+static bool mult_commutes(int32_t mult_op)
+{
+    return mult_op == GM_MULT_TIMES || mult_op == GM_MULT_PLUS || mult_op == GM_MULT_PAIR ||
+           mult_op == GM_MULT_K;
+}
+
+extern "C" bool gm_AxB_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
+                                   const gm_csr *B, const gm_semiring *sr, bool flipxy)
+{
+    if (!sr || (flipxy && !mult_commutes(sr->mult_op))) return false;
+    return gm_dot3_branch(M, mask_struct, A, B, sr);
+}
+
+extern "C" int gm_AxB_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                           const gm_csr *B, const gm_semiring *sr, bool flipxy)
+{
+    if (!sr) return set_error(GM_INVALID_VALUE, "gm_AxB_dot3: NULL semiring");
+    if (flipxy && !mult_commutes(sr->mult_op))
+        return set_error(GM_NO_VALUE, "gm_AxB_dot3: flipxy with a multiplier that does not commute");
+    return gm_dot3(C, M, mask_struct, A, B, sr, nullptr);
+}
+
 extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
                        const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
 {

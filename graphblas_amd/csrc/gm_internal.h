@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <type_traits>
+#include <vector>
 #include "gm_kernels.h"
 
 //------------------------------------------------------------------------------
@@ -45,6 +46,27 @@ struct DevBuf
 
 // pinned host staging for small device->host reads
 void *pinned_scratch(size_t bytes);
+
+// One piece of a stacked operand: rows [0, nrows) of the CSR (p, j, x), whose
+// row pointers may start anywhere (a slice of a larger matrix), placed at
+// stacked row row_base with col_add added to every column index.
+struct BatchPart
+{
+    const int64_t *p, *j;
+    const void *x;
+    int64_t nrows;
+    int64_t row_base;       // first stacked row
+    int64_t col_add;        // added to every column index
+    int64_t ent_base;       // first stacked entry (filled on the device)
+    int32_t iso;
+    int32_t pad;
+};
+
+// Stacks the parts into one CSR of total_rows rows on the library stream
+// (P, J and X receive the arrays; values of `type` when `values`, iso parts
+// expanded).  One device-to-host read of the total entry count.
+int bt_stack(const std::vector<BatchPart> &hparts, int64_t total_rows, int64_t ncols, bool values,
+             int32_t type, DevBuf &dparts, DevBuf &P, DevBuf &J, DevBuf &X, gm_csr *out);
 
 // exclusive scan of n int64 values in d_in into d_out (d_out[n] = total);
 // d_in and d_out may alias.  Uses the library stream.

@@ -4,10 +4,14 @@
 // Every host thread that calls the library gets its own non-blocking stream
 // at default priority (the initialising thread's is created by gm_init), so
 // concurrent callers -- the request server's workers -- overlap on the device.
+// A caller may put its own stream in place of the thread's library stream
+// (gm_set_stream), so the library's work is ordered with the caller's.
 // The arena keeps freed blocks in size-keyed free lists (HBM is large;
 // hipMalloc/hipFree are slow and synchronising), tagged with the stream that
-// freed them: a block is reused at once on its own stream (stream order makes
-// that safe) and after synchronising its stream on another one.  Blocks go
+// freed them and an event recorded there at the free: a block is reused at
+// once on its own stream (stream order makes that safe), and on another
+// stream after that stream is made to wait for the event on the device (no
+// host synchronisation, so concurrent callers do not serialise).  Blocks go
 // back to the driver on allocation failure or gm_arena_trim().
 
 #include "gm_internal.h"
@@ -29,7 +33,9 @@ static std::mutex g_mu;
 static int g_init_rc = 1;             // 1: not yet initialised
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;               // the initialising thread's stream
-static thread_local hipStream_t t_stream = nullptr;  // this thread's stream
+static thread_local hipStream_t t_stream = nullptr;  // this thread's library stream
+static thread_local hipStream_t t_user = nullptr;    // a caller's stream (gm_set_stream)
+static thread_local bool t_user_set = false;
 static std::vector<hipStream_t> g_thread_streams;    // created for other threads
 static int g_cu = 0;
 static int64_t g_hbm = 0;
@@ -54,6 +60,7 @@ int check_hip(hipError_t e, const char *what)
 
 hipStream_t stream()
 {
+    if (t_user_set) return t_user;
     if (t_stream) return t_stream;
     hipStream_t s = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return g_stream;
@@ -110,7 +117,7 @@ bool csr_ok(const gm_csr *X)
 // arena
 //------------------------------------------------------------------------------
 
-struct FreeBlock { void *p; hipStream_t s; };
+struct FreeBlock { void *p; hipStream_t s; hipEvent_t ev; };
 static std::multimap<size_t, FreeBlock> g_free;     // size -> block, freeing stream
 static std::unordered_map<void *, size_t> g_live;   // block -> size
 static size_t g_cached_bytes = 0, g_live_bytes = 0;
@@ -129,9 +136,15 @@ static size_t round_size(size_t b)
     return (b + g - 1) / g * g;
 }
 
+static std::vector<hipEvent_t> g_free_events;        // recycled free-block events
+
 static void trim_locked()
 {
-    for (auto &kv : g_free) (void) hipFree(kv.second.p);
+    for (auto &kv : g_free)
+    {
+        (void) hipFree(kv.second.p);
+        if (kv.second.ev) g_free_events.push_back(kv.second.ev);
+    }
     g_free.clear();
     g_cached_bytes = 0;
 }
@@ -425,6 +438,8 @@ void gm_finalize(void)
     if (g_init_rc != GM_SUCCESS) return;
     (void) hipStreamSynchronize(g_stream);
     trim_locked();
+    for (auto e : g_free_events) (void) hipEventDestroy(e);
+    g_free_events.clear();
     for (auto &kv : g_live) (void) hipFree(kv.first);
     g_live.clear();
     g_live_bytes = 0;
@@ -444,6 +459,31 @@ void gm_finalize(void)
 }
 
 const char *gm_last_error(void) { return g_err; }
+
+int gm_set_stream(void *s)
+{
+    GM_TRY(ensure_init());
+    if (s == nullptr)
+    {
+        t_user = nullptr;
+        t_user_set = false;
+        return GM_SUCCESS;
+    }
+    hipStream_t hs = (hipStream_t) s;
+    // reject a handle the runtime does not know (query returns its state)
+    const hipError_t q = hipStreamQuery(hs);
+    if (q != hipSuccess && q != hipErrorNotReady)
+    {
+        (void) hipGetLastError();
+        return set_error(GM_INVALID_VALUE, "gm_set_stream: not a stream of this device (%s)",
+                         hipGetErrorString(q));
+    }
+    t_user = hs;
+    t_user_set = true;
+    return GM_SUCCESS;
+}
+
+void *gm_get_stream(void) { return (void *) stream(); }
 
 int gm_device_count(void)
 {
@@ -485,7 +525,7 @@ void *gm_malloc(size_t bytes)
         g_fail_after--;
     }
     size_t r = round_size(bytes);
-    hipStream_t me = t_stream ? t_stream : g_stream;
+    hipStream_t me = stream();
     auto best = g_free.end();
     for (auto it = g_free.lower_bound(r); it != g_free.end() && it->first <= 2 * r; ++it)
     {
@@ -496,8 +536,10 @@ void *gm_malloc(size_t bytes)
     {
         void *p = best->second.p;
         const size_t sz = best->first;
-        // a block freed on another stream may still be in use there
-        if (best->second.s != me) (void) hipStreamSynchronize(best->second.s);
+        // a block freed on another stream may still be in use there: this
+        // stream waits on the device for the work queued before the free
+        if (best->second.s != me && best->second.ev) (void) hipStreamWaitEvent(me, best->second.ev, 0);
+        if (best->second.ev) g_free_events.push_back(best->second.ev);
         g_free.erase(best);
         g_cached_bytes -= sz;
         g_live[p] = sz;
@@ -533,7 +575,18 @@ void gm_free(void *p)
     size_t sz = it->second;
     g_live.erase(it);
     g_live_bytes -= sz;
-    g_free.emplace(sz, FreeBlock{p, t_stream ? t_stream : g_stream});
+    hipStream_t st = stream();
+    hipEvent_t ev = nullptr;
+    if (!g_free_events.empty()) { ev = g_free_events.back(); g_free_events.pop_back(); }
+    else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    if (ev && hipEventRecord(ev, st) != hipSuccess)
+    {
+        (void) hipGetLastError();
+        (void) hipEventDestroy(ev);
+        ev = nullptr;
+    }
+    if (!ev) (void) hipStreamSynchronize(st);     // no event: the block is idle once st drains
+    g_free.emplace(sz, FreeBlock{p, st, ev});
     g_cached_bytes += sz;
 }
 
@@ -547,6 +600,12 @@ int gm_arena_trim(void)
 }
 
 int64_t gm_arena_bytes_in_use(void) { return (int64_t) g_live_bytes; }
+
+int64_t gm_arena_bytes_cached(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return (int64_t) g_cached_bytes;
+}
 
 void gm_debug_fail_after(int64_t n)
 {

@@ -36,8 +36,10 @@ GB_method GB_mxm_select(bool has_mask, bool mask_comp, bool a_tran, bool b_tran,
 // this device the masked dot product on B' gives the same entries with work
 // about nnz(M) * (mean |A(i,:)| + mean |B(:,j)|) plus a sort of B for the
 // transpose, so dot3 is taken when that is well below the flop count (the
-// triangle count C<L>=L*L, for example).  Each C(i,j) sums the same terms in
-// the same k order either way.
+// triangle count C<L>=L*L, for example).  Each C(i,j) sums the same terms
+// A(i,k)*B(k,j) either way, so integer and boolean results are identical;
+// fp64 sums may round differently (dot3's teams add strided per-lane partials
+// and combine them by a tree, saxpy3 adds in k order).
 bool GB_mxm_prefer_dot3(int64_t flops, int64_t nnz_m, int64_t nnz_a, int64_t m, int64_t nnz_b,
                         int64_t n)
 {
@@ -245,6 +247,16 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
                        Mask ? "<!M>" : "", (long long) anvec, (long long) bnvec);
             method = GB_METHOD_DOT2;
         }
+    }
+    if (method == GB_METHOD_DOT3 && (Mask->nvals >= ((int64_t) 1 << 32) || Mask->nrows >= ((int64_t) 1 << 32)))
+    {
+        // the dot3 kernels number mask entries and rows with 32 bits: a larger
+        // mask takes saxpy3 on the rows of B' (the same entries, by the flop
+        // order of C<M>=A*(B')')
+        if (GB_burble)
+            printf("burble: GrB_mxm: mask of %lld entries beyond dot3's 2^32: saxpy3\n",
+                   (long long) Mask->nvals);
+        method = GB_METHOD_SAXPY3;
     }
     if (b_tran != (method == GB_METHOD_DOT3 || method == GB_METHOD_DOT2))
     {

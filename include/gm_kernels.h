@@ -10,9 +10,14 @@
 // and shares one caching device arena between them.
 //
 // Reference interfaces replaced (SuiteSparse:GraphBLAS v9.3.1):
-//   gm_dot3_branch / gm_dot3     <- GB_cuda_AxB_dot3_branch / GB_cuda_AxB_dot3
-//                                   (Source/gateway/GB_cuda_gateway.h:237-255,
-//                                   called from Source/mxm/GB_AxB_dot.c:189-197)
+//   gm_AxB_dot3_branch /         <- GB_cuda_AxB_dot3_branch / GB_cuda_AxB_dot3
+//   gm_AxB_dot3                     with the same argument list, flipxy included
+//                                   (Source/gateway/GB_cuda_gateway.h:237-256,
+//                                   called from Source/mxm/GB_AxB_dot.c:189-197);
+//   gm_dot3_branch / gm_dot3        the same pair with tuning knobs in place of
+//                                   flipxy
+//   gm_set_stream                <- the per-call stream of CUDA/GB_cuda_AxB_dot3.cpp
+//                                   :52-54, owned by the caller instead
 //   gm_saxpy3_branch / gm_saxpy3 <- GB_AxB_saxpy3 (Source/mxm/GB_AxB_saxpy3.c
 //                                   :119-700; the reference has no GPU saxpy3
 //                                   hook, so this mirrors the dot3 gateway pair)
@@ -158,10 +163,22 @@ void *gm_malloc(size_t bytes);      // device memory from the caching arena
 void gm_free(void *p);              // return to the arena (NULL ok)
 int gm_arena_trim(void);            // give cached blocks back to the driver
 int64_t gm_arena_bytes_in_use(void);
+int64_t gm_arena_bytes_cached(void);   // freed blocks the arena holds for reuse
 // host-forced allocation failure (tests): the n-th following gm_malloc
 // fails; n < 0 disables.  Mirrors the malloc_debug count-down of
 // Source/global/GB_Global.c:818-850.
 void gm_debug_fail_after(int64_t n);
+
+// The stream this thread's library calls run on.  gm_set_stream(s) makes
+// every later call of this thread enqueue its work on the caller's stream s
+// (a hipStream_t of the current device), ordered with the caller's own work;
+// gm_set_stream(NULL) restores the thread's library stream.  Calls that
+// return host results (counts, nvals) still wait for their own work.  The
+// reference's CUDA gateway makes a stream per call instead
+// (CUDA/GB_cuda_AxB_dot3.cpp:52-54); a caller-owned stream lets a
+// reference-side caller order the product with its other GPU work.
+int gm_set_stream(void *stream);
+void *gm_get_stream(void);
 
 int gm_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int gm_memcpy_d2h(void *dst, const void *src, size_t bytes);
@@ -194,10 +211,30 @@ int gm_prof_get(gm_prof_record *rec, int cap);
 
 // In CSR terms: for every entry (i,j) of M whose mask value allows it,
 // C(i,j) = (+)_k A(i,k) (x) B(j,k) over the k present in both rows.
+// The branch is a host-only test of shapes, types and the semiring.  The
+// kernels number mask entries and rows with 32 bits: gm_dot3 declines
+// (GM_NO_VALUE) a mask with 2^32 or more entries, and GrB_mxm, which knows
+// nnz(M) on the host, sends such a product to saxpy3 on B' instead.
 bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A,
                     const gm_csr *B, const gm_semiring *sr);
 int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
             const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params);
+
+// The reference's gateway pair with its argument list
+// (GB_cuda_AxB_dot3_branch / GB_cuda_AxB_dot3,
+// Source/gateway/GB_cuda_gateway.h:237-256, called at
+// Source/mxm/GB_AxB_dot.c:189-197): flipxy asks for z = mult(B(j,k), A(i,k))
+// in place of mult(A(i,k), B(j,k)).  Every multiplier of the built-in
+// semirings here is unchanged by the swap -- TIMES and PLUS commute (also
+// bit for bit in IEEE fp64), PAIR is a constant, and the positional
+// multiplier is the inner index k whichever operand comes first (the
+// reference's GB_flip_binop turns FIRSTJ into SECONDI, both k here,
+// Source/binaryop/GB_flip_binop.c:351-373) -- so flipxy selects the same
+// kernels; a multiplier that does not commute would be declined.
+bool gm_AxB_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *A, const gm_csr *B,
+                        const gm_semiring *sr, bool flipxy);
+int gm_AxB_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                const gm_csr *B, const gm_semiring *sr, bool flipxy);
 
 // dot2: C = A*B' or C<!M> = A*B' (M may be NULL; a mask that is not
 // complemented goes to gm_dot3) with every dot product of a nonempty row of A
@@ -278,39 +315,63 @@ int gm_batch_split(gm_csr_out *out, const gm_csr_out *C, const int64_t *row_base
 // that receive many small BFS-step requests against one resident graph)
 //------------------------------------------------------------------------------
 
-// A request is C<!V> = F*A on the server's graph A (square, in HBM): F has
-// one row per source (F(r, src[r]) = 1, iso), V is the request's visited set
-// (k rows; V = F when none is given).  With GxB_ANY_SECONDI_INT64, C(r,i) is
-// src[r] for each newly reached vertex i, the reference's BFS step.  Worker
-// threads, each on its own HIP stream, admit pending requests in arrival
-// order while the batch's flop bound (sum of deg(src)) stays within
-// flop_budget, and run the admitted batch as one stacked saxpy3 call.
+// A request is a breadth-first search of one or more levels on the server's
+// graph A (square, in HBM): F has one row per source (F(r, src[r]) = 1, iso)
+// and V is the request's visited set (k rows; V = F when none is given).
+// Each level is C<!V> = F*A; with GxB_ANY_SECONDI_INT64, C(r,i) is the
+// parent of every newly reached vertex i, the reference's BFS step
+// (GraphBLAS/@GrB/bfs.m:140,162).  F and V stay in HBM between levels: the
+// next frontier is C and V grows by C (V's values are the parents: the
+// source for its own row, -1 for vertices the caller marked visited).
+// Worker threads, each on its own HIP stream, admit pending requests in
+// arrival order while the batch's workspace estimate stays within the byte
+// budget (the lesser of mem_budget and half the HBM still free) and its flop
+// bound (sum of deg over the frontier entries) within flop_budget, and run
+// the admitted level of every request as one stacked saxpy3 call.  A request
+// with levels left goes back to the front of the queue and meets the
+// requests that arrived meanwhile in the next batch.  A batch that runs out
+// of device memory is queued again and the next admissions take at most
+// half as many requests; a single request that does not fit fails with
+// GM_OUT_OF_MEMORY.
 typedef struct
 {
-    int32_t workers;         // worker threads (2)
+    int32_t workers;         // worker threads (2); 0: none, the caller runs each
+                             // batch on its own thread with gm_server_step
     int32_t max_batch_reqs;  // requests per batch at most (256)
     int64_t flop_budget;     // flop bound per batch (2^24); a larger request runs alone
-    int64_t reserved;
+    int64_t mem_budget;      // workspace bytes per batch (0: a quarter of HBM)
 } gm_server_config;
+
+typedef struct
+{
+    int64_t max_steps;       // levels to run: 1 (the default), or <= 0 until the
+                             // frontier is empty
+    int32_t keep_visited;    // keep V for gm_server_fetch_visited
+    int32_t pad;
+} gm_server_request_opts;
 
 typedef struct
 {
     int32_t status;          // GM_SUCCESS or the batch's error code
     int32_t pad;
     int64_t k;               // rows (sources) of the request
-    int64_t nvals;           // entries of the request's result
-    int64_t flops;           // its flop bound
+    int64_t nvals;           // entries of the last level's result
+    int64_t flops;           // flop bounds of the levels run
     double latency_us;       // arrival -> result complete
-    double queue_us;         // arrival -> its batch started
-    int64_t batch_id;        // batch that computed it
+    double queue_us;         // arrival -> its first batch started
+    int64_t batch_id;        // batch that computed its last level
     int64_t batch_size;      // requests in that batch
+    int64_t steps;           // levels run
+    int64_t vnvals;          // entries of V (when kept)
 } gm_server_result;
 
 typedef struct
 {
     int64_t completed, batches, pending, max_in_flight;
     double mean_batch;       // requests per batch
-    double flops;            // flop bound of completed requests
+    double flops;            // flop bound of the levels completed
+    int64_t levels;          // request levels completed
+    int64_t oom_splits;      // batches queued again after running out of memory
 } gm_server_counters;
 
 typedef struct gm_server gm_server;
@@ -320,16 +381,28 @@ typedef struct gm_server gm_server;
 gm_server *gm_server_create(const gm_csr *graph, const gm_semiring *sr,
                             const gm_server_config *cfg);
 // sources: k host vertex ids; vis_p (k+1) / vis_j: optional host CSR of the
-// visited set (both NULL: V = F).  Returns the request id (> 0) or an error
-// code (< 0).
+// visited set (both NULL: V = F; vis_p[0] = 0, rows sorted, columns in
+// [0, n)), uploaded to HBM here once.  opts may be NULL (one level).
+// Returns the request id (> 0) or an error code (< 0).
+int64_t gm_server_submit_bfs(gm_server *s, const int64_t *sources, int64_t k,
+                             const int64_t *vis_p, const int64_t *vis_j,
+                             const gm_server_request_opts *opts);
+// one level (gm_server_submit_bfs with opts NULL)
 int64_t gm_server_submit(gm_server *s, const int64_t *sources, int64_t k,
                          const int64_t *vis_p, const int64_t *vis_j);
-// blocks until the request is complete; fills *out
+// a server without worker threads: admits one batch and runs its level on
+// the calling thread; returns the batch's request count (0: none pending)
+int64_t gm_server_step(gm_server *s);
+// blocks until the request is complete; fills *out (without worker threads
+// a request that is not complete is an error)
 int gm_server_wait(gm_server *s, int64_t id, gm_server_result *out);
-// copies the request's rows (k+1 row pointers from 0, nvals indices and
-// values; Cj / Cx may be NULL) to host memory
+// copies the request's last level's rows (k+1 row pointers from 0, nvals
+// indices and values; Cj / Cx may be NULL) to host memory
 int gm_server_fetch(gm_server *s, int64_t id, int64_t *Cp, int64_t *Cj, void *Cx);
-// forgets the request (its HBM result goes with its batch's last request)
+// copies the request's visited set (k+1 row pointers, vnvals indices and
+// parents) when it was submitted with keep_visited
+int gm_server_fetch_visited(gm_server *s, int64_t id, int64_t *Vp, int64_t *Vj, void *Vx);
+// forgets the request (its HBM results go with the last request that holds them)
 int gm_server_release(gm_server *s, int64_t id);
 int gm_server_stats(gm_server *s, gm_server_counters *st);
 // drains pending requests, joins the workers, frees everything

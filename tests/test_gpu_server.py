@@ -131,3 +131,132 @@ def test_server_drive_poisson(grb):
     finally:
         srv.close()
         dG.free()
+
+
+def _oracle_bfs(G, src, max_steps):
+    """Levels of C<!V> = F*A (any_secondi int64) with V += C, F = C until the
+    frontier is empty or max_steps levels ran; returns (last C, V with parents,
+    levels run).  The oracle's saxpy3 witness is the last writer in k order,
+    as on the device."""
+    k = len(src)
+    F = W.Csr(k, G.nrows, np.arange(k + 1, dtype=np.int64), np.asarray(src, dtype=np.int64))
+    parents = [{int(s): int(s)} for s in src]
+    V = F
+    C = None
+    steps = 0
+    while True:
+        C = O.saxpy3("any_secondi_int64", F, G, M=V, mask_comp=True)
+        steps += 1
+        for r in range(k):
+            for q in range(C.p[r], C.p[r + 1]):
+                parents[r][int(C.j[q])] = int(C.x[q])
+        vp = np.zeros(k + 1, dtype=np.int64)
+        vj, vx = [], []
+        for r in range(k):
+            keys = sorted(parents[r])
+            vj += keys
+            vx += [parents[r][c] for c in keys]
+            vp[r + 1] = len(vj)
+        V = W.Csr(k, G.nrows, vp, np.array(vj, dtype=np.int64), np.array(vx, dtype=np.int64))
+        if C.nnz == 0 or (max_steps > 0 and steps >= max_steps):
+            return C, V, steps
+        F = C
+
+
+@pytest.mark.parametrize("workers", [1, 2])
+def test_server_multilevel_bfs_matches_oracle(grb, workers):
+    """BFS requests of several levels (and until the frontier is empty) with
+    their frontier and visited set kept in HBM between levels, mixed with
+    one-level requests: the last level and the visited set with parents
+    equal the oracle's level-by-level loop."""
+    print(f"[server] multilevel bfs workers={workers}", flush=True)
+    G = W.rmat_sym(12, 8, seed=5)
+    dG = DevCsr(G, 1)
+    rng = np.random.default_rng(21 + workers)
+    srv = capi.Server(dG.view, ANY_SECONDI, workers=workers, max_batch_reqs=16)
+    try:
+        reqs = []
+        for t in range(24):
+            k = int(rng.integers(1, 9))
+            src = np.sort(rng.choice(G.nrows, size=k, replace=False)).astype(np.int64)
+            steps = [1, 2, 3, 0][t % 4]
+            reqs.append((src, steps, srv.submit(src, max_steps=steps, keep_visited=True)))
+        for src, steps, rid in reqs:
+            res = srv.wait(rid)
+            assert res.status == 0, capi.last_error()
+            C, V, nst = _oracle_bfs(G, src, steps)
+            assert res.steps == nst, (steps, res.steps, nst)
+            p, j, x = srv.fetch(rid, res)
+            assert_csr_equal(W.Csr(len(src), G.ncols, p, j, x), C, what=f"last level of {rid}")
+            vp, vj, vx = srv.fetch_visited(rid, res)
+            assert_csr_equal(W.Csr(len(src), G.ncols, vp, vj, vx), V, what=f"visited set of {rid}")
+            srv.release(rid)
+        st = srv.stats()
+        assert st["levels"] >= sum(1 for _ in reqs)
+    finally:
+        srv.close()
+        dG.free()
+
+
+def test_server_out_of_memory_splits_and_requeues(grb):
+    """An allocation forced to fail in the middle of a batch (the count-down
+    injector) sends the batch back to the queue with a halved cap; every
+    request still equals the oracle.  No worker threads, so the failure lands
+    in a known batch."""
+    G = W.rmat_sym(11, 16, seed=9)
+    dG = DevCsr(G, 1)
+    L = capi.lib()
+    reqs = [s for s, _ in _requests(G.nrows, 40, seed=13, vis_frac=0.0)]
+    srv = capi.Server(dG.view, ANY_SECONDI, workers=0, max_batch_reqs=64)
+    try:
+        ids = [srv.submit(s) for s in reqs]
+        L.gm_debug_fail_after(3)         # the 4th allocation of the first batch fails
+        try:
+            first = srv.step()
+        finally:
+            L.gm_debug_fail_after(-1)
+        assert first == 40               # admitted whole, then queued again
+        assert srv.stats()["oom_splits"] == 1
+        sizes = []
+        while True:
+            n = srv.step()
+            if n == 0:
+                break
+            sizes.append(n)
+        assert sizes[0] == 20            # the next admission takes half
+        assert sum(sizes) == 40
+        for src, rid in zip(reqs, ids):
+            res = srv.wait(rid)
+            assert res.status == 0
+            p, j, x = srv.fetch(rid, res)
+            k = len(src)
+            F = W.Csr(k, G.nrows, np.arange(k + 1, dtype=np.int64), src)
+            ref = O.saxpy3("any_secondi_int64", F, G, M=F, mask_comp=True)
+            assert_csr_equal(W.Csr(k, G.ncols, p, j, x), ref, what=f"request {rid}")
+            srv.release(rid)
+    finally:
+        srv.close()
+        dG.free()
+
+
+def test_server_rejects_bad_visited_sets(grb):
+    G = W.rmat_sym(8, 4, seed=1)
+    dG = DevCsr(G, 1)
+    srv = capi.Server(dG.view, ANY_SECONDI, workers=1)
+    try:
+        src = np.array([1, 2], dtype=np.int64)
+        bad = [
+            (np.array([1, 1, 2]), np.array([3, 4])),           # vis_p[0] != 0
+            (np.array([0, 2, 1]), np.array([3, 4])),           # not monotone
+            (np.array([0, 1, 2]), np.array([3, G.nrows])),     # column out of range
+            (np.array([0, 2, 2]), np.array([5, 4])),           # row not sorted
+            (np.array([0, 2, 2]), np.array([4, 4])),           # duplicate column
+        ]
+        for vp, vj in bad:
+            with pytest.raises(capi.GrBError):
+                srv.submit(src, (vp, vj))
+        rid = srv.submit(src, (np.array([0, 1, 2]), np.array([1, 2])))
+        assert srv.wait(rid).status == 0
+    finally:
+        srv.close()
+        dG.free()
