@@ -242,6 +242,7 @@ def lib():
                        ctypes.c_bool, ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr),
                        ctypes.POINTER(GmSemiring), ctypes.POINTER(GmSaxpy3Params)], ctypes.c_int),
         "gm_reduce_scalar": ([P, I32, I32, P, I64, I32], ctypes.c_int),
+        "gm_reduce_scalar_noalloc": ([P, I32, I32, P, I64, I32, P], ctypes.c_int),
         "gm_batch_dot3": ([ctypes.POINTER(GmCsrOut), P, P, ctypes.POINTER(GmDot3Request), I64,
                            ctypes.c_bool, ctypes.POINTER(GmSemiring),
                            ctypes.POINTER(GmDot3Params)], ctypes.c_int),

@@ -17,6 +17,8 @@ namespace gm
 
 hipStream_t stream();                       // the library stream
 int ensure_init();                          // gm_init() result, cached
+bool is_initialised();                      // gm_init() has succeeded
+int reduce_scratch_setup();                 // device address of the reduction scratch (gm_reduce.hip)
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 int cu_count();

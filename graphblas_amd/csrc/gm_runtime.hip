@@ -80,6 +80,8 @@ int ensure_init()
     return gm_init();
 }
 
+bool is_initialised() { return g_init_rc == GM_SUCCESS; }
+
 size_t type_size(int32_t type)
 {
     switch (type)
@@ -408,6 +410,16 @@ static int gm_init_device(void)
     if (hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking) != hipSuccess)
     {
         g_init_rc = set_error(GM_PANIC, "gm_init: stream creation failed");
+        return g_init_rc;
+    }
+    // the allocation-free entry points' scratch: its device address, looked
+    // up once with this device current
+    const int src = reduce_scratch_setup();
+    if (src != GM_SUCCESS)
+    {
+        (void) hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+        g_init_rc = src;
         return g_init_rc;
     }
     g_device = dev;

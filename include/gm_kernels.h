@@ -417,6 +417,21 @@ void gm_server_destroy(gm_server *s);
 int gm_reduce_scalar(void *result, int32_t add_op, int32_t type, const void *x,
                      int64_t n, int32_t iso);
 
+// Allocation-free form of gm_reduce_scalar.  No counterpart in the reference:
+// it is for callers that cannot allocate device or pinned memory (code under
+// HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+// result: DEVICE pointer to one value of `type`, written on `stream`
+// (NULL: the calling thread's library stream); nothing is copied to the host
+// and nothing is waited for, so read it after synchronising the stream.
+// The first pass's partials live in a fixed 8 KB device array per GPU (one
+// value per workgroup of a fixed grid, so every n fits; the array is held
+// until the process ends).  gm_init() must have been called (it looks up the
+// array's address); otherwise GM_PANIC.  Calls that overlap on one GPU (on
+// different streams) race on the array.  Same results as gm_reduce_scalar
+// (the same kernels combine the same partials in the same order).
+int gm_reduce_scalar_noalloc(void *result, int32_t add_op, int32_t type, const void *x,
+                             int64_t n, int32_t iso, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
