@@ -74,6 +74,19 @@ class GrBError(RuntimeError):
 class GmCsr(ctypes.Structure):
     _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64),
                 ("p", ctypes.c_void_p), ("j", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("iso", ctypes.c_int32), ("type", ctypes.c_int32),
+                ("aux", ctypes.c_void_p)]
+
+
+class GmDense(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64),
+                ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("iso", ctypes.c_int32), ("type", ctypes.c_int32)]
+
+
+class GmDenseOut(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("ncols", ctypes.c_int64), ("nvals", ctypes.c_int64),
+                ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
                 ("iso", ctypes.c_int32), ("type", ctypes.c_int32)]
 
 
@@ -243,6 +256,24 @@ def lib():
                        ctypes.POINTER(GmSemiring), ctypes.POINTER(GmSaxpy3Params)], ctypes.c_int),
         "gm_reduce_scalar": ([P, I32, I32, P, I64, I32], ctypes.c_int),
         "gm_reduce_scalar_noalloc": ([P, I32, I32, P, I64, I32, P], ctypes.c_int),
+        "gm_csr_aux_build": ([PP, ctypes.POINTER(GmCsr)], ctypes.c_int),
+        "gm_dense_to_csr": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmDense)], ctypes.c_int),
+        "gm_csr_to_dense": ([ctypes.POINTER(GmDenseOut), ctypes.POINTER(GmCsr), ctypes.c_bool],
+                            ctypes.c_int),
+        "gm_saxbit": ([ctypes.POINTER(GmDenseOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmDense),
+                       ctypes.c_bool, ctypes.c_bool, ctypes.POINTER(GmCsr), ctypes.POINTER(GmDense),
+                       ctypes.POINTER(GmCsr), ctypes.POINTER(GmDense), ctypes.POINTER(GmSemiring),
+                       ctypes.POINTER(GmDense)], ctypes.c_int),
+        "GxB_Matrix_Option_set_INT32": ([P, ctypes.c_int, I32], ctypes.c_int),
+        "GxB_Matrix_Option_get_INT32": ([P, ctypes.c_int, ctypes.POINTER(I32)], ctypes.c_int),
+        "GxB_Matrix_pack_BitmapR": ([P, PP, PP, U64, U64, ctypes.c_bool, U64, P], ctypes.c_int),
+        "GxB_Matrix_pack_FullR": ([P, PP, U64, ctypes.c_bool, P], ctypes.c_int),
+        "GxB_Matrix_unpack_BitmapR": ([P, PP, PP, ctypes.POINTER(U64), ctypes.POINTER(U64),
+                                       ctypes.POINTER(ctypes.c_bool), ctypes.POINTER(U64), P],
+                                      ctypes.c_int),
+        "GxB_Matrix_unpack_FullR": ([P, PP, ctypes.POINTER(U64), ctypes.POINTER(ctypes.c_bool), P],
+                                    ctypes.c_int),
+        "gm_csr_aux_free": ([P], None),
         "gm_batch_dot3": ([ctypes.POINTER(GmCsrOut), P, P, ctypes.POINTER(GmDot3Request), I64,
                            ctypes.c_bool, ctypes.POINTER(GmSemiring),
                            ctypes.POINTER(GmDot3Params)], ctypes.c_int),
@@ -413,6 +444,70 @@ class Matrix:
         v = ctypes.c_bool()
         _check(lib().GxB_Matrix_iso(ctypes.byref(v), self.h), "iso", self.h)
         return v.value
+
+    # storage formats (GxB_SPARSITY_STATUS / GxB_SPARSITY_CONTROL)
+    SPARSE, BITMAP, FULL = 2, 4, 8
+
+    @property
+    def sparsity(self) -> int:
+        v = ctypes.c_int32()
+        _check(lib().GxB_Matrix_Option_get_INT32(self.h, 7034, ctypes.byref(v)), "sparsity", self.h)
+        return v.value
+
+    def set_sparsity_control(self, ctrl: int):
+        _check(lib().GxB_Matrix_Option_set_INT32(self.h, 7036, ctrl), "sparsity control", self.h)
+
+    @classmethod
+    def from_dense(cls, x: np.ndarray, present: Optional[np.ndarray], type_name: str) -> "Matrix":
+        """Pack a row-major dense array as a bitmap (present given) or full
+        matrix (present None)."""
+        nrows, ncols = x.shape
+        M = cls(type_name, nrows, ncols)
+        dt = NP_TYPES[type_name]
+        xv = np.ascontiguousarray(x, dtype=dt)
+        xp = ctypes.c_void_p(_malloc_copy(xv))
+        if present is None:
+            rc = lib().GxB_Matrix_pack_FullR(M.h, ctypes.byref(xp), max(xv.nbytes, 1), False, None)
+            bp = None
+        else:
+            bv = np.ascontiguousarray(present, dtype=np.int8)
+            bp = ctypes.c_void_p(_malloc_copy(bv))
+            rc = lib().GxB_Matrix_pack_BitmapR(M.h, ctypes.byref(bp), ctypes.byref(xp),
+                                               max(bv.nbytes, 1), max(xv.nbytes, 1), False,
+                                               int(np.count_nonzero(bv)), None)
+        if rc != GrB_SUCCESS:
+            for q in (xp, bp):
+                if q is not None and q.value:
+                    _libc.free(q)
+        _check(rc, "GxB_Matrix_pack", M.h)
+        return M
+
+    def to_dense(self):
+        """(values, present) as row-major arrays (the matrix is unchanged)."""
+        D = ctypes.c_void_p()
+        _check(lib().GrB_Matrix_dup(ctypes.byref(D), self.h), "GrB_Matrix_dup", self.h)
+        bp, xp = ctypes.c_void_p(), ctypes.c_void_p()
+        bs, xs, nv = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        iso = ctypes.c_bool()
+        rc = lib().GxB_Matrix_unpack_BitmapR(D, ctypes.byref(bp), ctypes.byref(xp), ctypes.byref(bs),
+                                             ctypes.byref(xs), ctypes.byref(iso), ctypes.byref(nv),
+                                             None)
+        try:
+            _check(rc, "GxB_Matrix_unpack_BitmapR", D)
+            mn = self.nrows * self.ncols
+            dt = NP_TYPES[self.type_name]
+            b = np.frombuffer(bytes((ctypes.c_uint8 * max(mn, 1)).from_address(bp.value)),
+                              dtype=np.int8)[:mn].copy()
+            nx = 1 if iso.value else max(mn, 1)
+            xb = (ctypes.c_uint8 * (nx * np.dtype(dt).itemsize)).from_address(xp.value)
+            x = np.frombuffer(bytes(xb), dtype=dt)
+            x = np.full(mn, x[0], dtype=dt) if iso.value else x[:mn].copy()
+            return x.reshape(self.nrows, self.ncols), b.reshape(self.nrows, self.ncols) != 0
+        finally:
+            for q in (bp, xp):
+                if q.value:
+                    _libc.free(q)
+            lib().GrB_Matrix_free(ctypes.byref(D))
 
     @classmethod
     def from_csr(cls, m: Csr, type_name: str, iso_value=None) -> "Matrix":

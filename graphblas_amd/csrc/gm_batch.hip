@@ -299,7 +299,7 @@ extern "C" int gm_batch_dot3(gm_csr_out *C, int64_t *row_base, int64_t *col_base
             if (!reqs[r].M.x) return set_error(GM_NO_VALUE, "gm_batch_dot3: valued mask without values");
 
     DevBuf tm, Pm, Jm, Xm, ta, Pa, Ja, Xa, tb, Pb, Jb, Xb;
-    gm_csr M, A, B;
+    gm_csr M{}, A{}, B{};
     if (shared_all)
     {
         // every request is C<A> = A*A': stack A once and use it three times
@@ -368,7 +368,7 @@ extern "C" int gm_batch_saxpy3(gm_csr_out *C, int64_t *row_base, const gm_saxpy3
     row_base[nreq] = rows;
     const bool valued = !(sr->mult_op == GM_MULT_PAIR || sr->mult_op == GM_MULT_K);
     DevBuf ta, Pa, Ja, Xa, tm, Pm, Jm, Xm;
-    gm_csr A, M;
+    gm_csr A{}, M{};
     GM_TRY(bt_stack(pa, rows, B->nrows, valued, valued ? reqs[0].A.type : GM_BOOL, ta, Pa, Ja, Xa, &A));
     if (masked)
     {

@@ -1247,6 +1247,179 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
 }
 
 //------------------------------------------------------------------------------
+// 16-lane bin against LDS hash tables of the long rows (pattern semirings)
+//------------------------------------------------------------------------------
+
+// The 16-lane bin sorted by probed long row, for operands whose rows span
+// more than L1_BITS columns (so a whole-row LDS bitmap does not fit).  The
+// bin is cut into slices of LH_SLICE entries.  In a slice, the runs of
+// entries with one key are found by a workgroup scan; a run whose long row
+// has a probe index, at most LH_MAX entries, and enough entries to pay for
+// a table (LH_GMIN, and 32 entries per LH_PAY entries of the long row) loads
+// that row's columns into an LDS hash table once (2^ceil(log2 2l) slots of
+// 32-bit columns, linear probing), and the teams probe their short rows
+// there: LDS reads in place of one random L2 / HBM access per element.  The
+// slice's other entries take the global path of the team kernel first (the
+// table's LDS doubles as their staging area then).  Slices are handed out
+// per XCD region as in the grouped team kernels.
+constexpr int LH_SLICE = 256;
+constexpr int LH_MAX = 2048;                   // long row entries at most
+constexpr int LH_SLOTS = 2 * LH_MAX;           // 16 KB of LDS
+constexpr int LH_GMIN = 4;
+constexpr int LH_PAY = 32;
+constexpr uint32_t LH_EMPTY = 0xffffffffu;
+static_assert(LH_SLOTS * sizeof(uint32_t) >= (D3_NT / 16) * StageCap<16>::value * sizeof(uint32_t),
+              "the table's LDS holds the global path's staging lists");
+
+template <class S, int G>
+__global__ __launch_bounds__(D3_NT) void k_dot3_team_lh(Dot3Args g, const uint64_t *list,
+                                                        const uint64_t *keys, int64_t count,
+                                                        unsigned long long *ctr)
+{
+    using Z = typename S::Z;
+    static_assert(S::pattern, "LDS hash probing covers the pattern semirings");
+    constexpr int TPB = D3_NT / G;
+    static_assert(LH_SLICE == D3_NT, "one slice entry per thread");
+    __shared__ uint32_t s_tab[LH_SLOTS];        // hash table, or the teams' staging lists
+    __shared__ uint64_t s_key[LH_SLICE];
+    __shared__ int s_rs[LH_SLICE + 1];          // start of run r in the slice
+    __shared__ uint8_t s_re[LH_SLICE];          // run r takes the LDS table
+    __shared__ int s_rid[LH_SLICE];             // run of entry t
+    __shared__ int s_scan[D3_NT / 64 + 1];
+    __shared__ int64_t s_slice;
+    const int lane = threadIdx.x & (G - 1), team = threadIdx.x / G;
+    const int64_t nslices = (count + LH_SLICE - 1) / LH_SLICE;
+    const int x0 = (int) (blockIdx.x & 7);
+    for (int k = 0; k < 8; k++)
+    {
+        const int r = (x0 + k) & 7;
+        const int64_t slo = nslices * r / 8, shi = nslices * (r + 1) / 8;
+        while (true)
+        {
+            if (threadIdx.x == 0) s_slice = slo + (int64_t) atomicAdd(&ctr[r], 1ull);
+            __syncthreads();
+            const int64_t sl_id = s_slice;
+            __syncthreads();
+            if (sl_id >= shi) break;
+            const int64_t q0 = sl_id * LH_SLICE;
+            const int n = (int) ((count - q0) < LH_SLICE ? (count - q0) : LH_SLICE);
+            // runs of equal keys: flag the first entry of each, scan for run ids
+            const int t = threadIdx.x;
+            const uint64_t kt = t < n ? keys[q0 + t] : ~0ull;
+            s_key[t] = kt;
+            __syncthreads();
+            const int first = (t < n && (t == 0 || s_key[t - 1] != kt)) ? 1 : 0;
+            int nruns;
+            const int rid = block_exclusive_scan<int, D3_NT>(first, s_scan, &nruns) + first - 1;
+            if (first)
+            {
+                s_rs[rid] = t;
+                s_rid[t] = rid;
+            }
+            if (t == 0) s_rs[nruns] = n;
+            __syncthreads();
+            if (t < n && !first) s_rid[t] = rid;
+            // run eligibility, decided by its first entry
+            if (first)
+            {
+                const int rlen = s_rs[rid + 1] - t;
+                const bool rowA = kt < (uint64_t) g.key_nA;
+                const int64_t rr = rowA ? (int64_t) kt : (int64_t) kt - g.key_nA;
+                const RowDesc d = rowA ? g.dA[rr] : g.dB[rr];
+                s_re[rid] = (rlen >= LH_GMIN && d.kind >= 0 && d.len > 0 && d.len <= LH_MAX &&
+                             (int64_t) rlen * LH_PAY >= (int64_t) d.len) ? 1 : 0;
+            }
+            __syncthreads();
+            // entries of the other runs: the team kernel's own path
+            for (int e = team; e < n; e += TPB)
+                if (!s_re[s_rid[e]]) team_task<S, G>(g, list[q0 + e], lane, s_tab + team * StageCap<G>::value);
+            for (int rr_ = 0; rr_ < nruns; rr_++)
+            {
+                if (!s_re[rr_]) continue;
+                const int gs = s_rs[rr_], ge = s_rs[rr_ + 1];
+                const uint64_t key = s_key[gs];
+                const bool rowA = key < (uint64_t) g.key_nA;
+                const int64_t rr = rowA ? (int64_t) key : (int64_t) key - g.key_nA;
+                const RowDesc d = rowA ? g.dA[rr] : g.dB[rr];
+                const int lg = ilog2_ceil(2 * (int64_t) d.len);
+                const uint32_t hmask = (1u << lg) - 1u;
+                __syncthreads();                  // previous users of s_tab are done
+                for (int w = threadIdx.x; w <= (int) hmask; w += D3_NT) s_tab[w] = LH_EMPTY;
+                __syncthreads();
+                {
+                    const int64_t *Lj = rowA ? g.Aj : g.Bj;
+                    for (int e = threadIdx.x; e < d.len; e += D3_NT)
+                    {
+                        const uint32_t x = (uint32_t) Lj[d.p0 + e];
+                        uint32_t h = hx_hash(x, lg);
+                        while (atomicCAS(&s_tab[h], LH_EMPTY, x) != LH_EMPTY) h = (h + 1) & hmask;
+                    }
+                }
+                __syncthreads();
+                for (int e = gs + team; e < ge; e += TPB)
+                {
+                    const uint64_t w = list[q0 + e];
+                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+                    Pair P;
+                    (void) d3_pair(g, i, g.Mj[p], P);
+                    bool ex = false;
+                    Z z = S::identity();
+                    if (!P.empty && P.probe == (rowA ? 1 : 2))
+                    {
+                        // the short row from its first element in the long
+                        // row's range, four elements per lane per step, the
+                        // lane stopping after a step past the range
+                        int64_t s0, s1;
+                        d3_trim_short_team<G>(g, P, lane, s0, s1);
+                        const int64_t *Sj = rowA ? g.Bj : g.Aj;
+                        for (int64_t qs = s0 + lane; qs < s1; qs += 4 * G)
+                        {
+                            int64_t xs[4];
+                            #pragma unroll
+                            for (int u = 0; u < 4; u++)
+                            {
+                                const int64_t qq = qs + u * G;
+                                xs[u] = qq < s1 ? (int64_t) Sj[qq] : INT64_MAX;
+                            }
+                            #pragma unroll
+                            for (int u = 0; u < 4; u++)
+                            {
+                                const int64_t x = xs[u];
+                                if (x > (int64_t) d.hi) break;
+                                uint32_t h = hx_hash((uint32_t) x, lg);
+                                bool hit = false;
+                                while (true)
+                                {
+                                    const uint32_t v = s_tab[h];
+                                    if (v == (uint32_t) x) { hit = true; break; }
+                                    if (v == LH_EMPTY) break;
+                                    h = (h + 1) & hmask;
+                                }
+                                if (!hit) continue;
+                                const Z tz = S::mul(Z(1), Z(1), x);
+                                if (!ex) { z = tz; ex = true; }
+                                else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                                else z = S::upd(z, tz);
+                            }
+                            if (xs[3] > (int64_t) d.hi) break;
+                        }
+                    }
+                    else if (!P.empty)
+                    {
+                        // an entry of this run that probes the other way
+                        // (its short row is the run's row): global path
+                        dot_lane<S, G>(g, P, lane, ex, z);
+                    }
+                    team_reduce<S, G>(ex, z);
+                    if (lane == 0) d3_store<S>(g, p, ex, z);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+//------------------------------------------------------------------------------
 // row mode: whole mask rows against an LDS bitmap of A(i,:)
 //------------------------------------------------------------------------------
 
@@ -1840,6 +2013,18 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
                     g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
             GM_HIP(hipGetLastError(), "k_dot3_lds1");
         }
+        else if (b == 0 && lds)
+        {
+            // LDS hash tables of the long rows that pay for one (pattern
+            // semirings); the rest of the bin on the global path
+            if constexpr (S::pattern)
+            {
+                ProfScope ps("dot3_team16_lh");
+                k_dot3_team_lh<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
+                GM_HIP(hipGetLastError(), "k_dot3_team_lh");
+            }
+        }
         else if (b == 0)
         {
             // short rows: LDS chunks cost more than they save here
@@ -2212,7 +2397,7 @@ static int dot2_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_st
             use_m ? M->iso : 0, mstruct, Fp.as<int64_t>(), Fj.as<int64_t>());
         GM_HIP(hipGetLastError(), "k_d2_fill");
     }
-    gm_csr F;
+    gm_csr F{};
     F.nrows = m;
     F.ncols = nb;
     F.p = Fp.as<int64_t>();

@@ -8,6 +8,14 @@
 #include <vector>
 #include "gm_kernels.h"
 
+// derived forms of a CSR operand (built by gm_csr_aux_build, gm_saxpy3.hip)
+struct gm_csr_aux
+{
+    int64_t nrows;
+    int32_t rec_lg;     // row records: row k at k << rec_lg (-1: none)
+    void *rec;          // nrows << rec_lg records of 16 bytes (column, value bits)
+};
+
 //------------------------------------------------------------------------------
 // runtime (gm_runtime.hip)
 //------------------------------------------------------------------------------

@@ -68,8 +68,11 @@ struct SxArgs
     uint64_t *glist;
     int64_t ws_max, wl_max, blk_max;
     // B's entries as 16-byte (column, value bits) pairs, or null: one B row
-    // of the gather then spans one contiguous range instead of two
+    // of the gather then spans one contiguous range instead of two.  With
+    // rec_lg >= 0 they are B's row records instead (row k at k << rec_lg,
+    // its length in the top byte of the first record's column)
     const int64_t *Bjx;
+    int32_t rec_lg;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -732,6 +735,31 @@ __global__ __launch_bounds__(256) void k_sx_pack(const int64_t *__restrict__ Bj,
         out[e] = make_longlong2(Bj[e], Bx[e]);
 }
 
+// Row records of B (gm_csr_aux): slot s of row k at (k << lg) + s holds
+// (column, value bits) of the row's s-th entry, or (0, 0) past its end; slot
+// 0's column carries the row length in its top byte (columns < 2^56)
+constexpr long long SX_REC_COL = (1ll << 56) - 1;
+constexpr int SX_REC_MAX_LG = 4;          // rows of at most 16 entries
+
+__global__ __launch_bounds__(256) void k_sx_rec_build(const int64_t *__restrict__ Bp,
+                                                      const int64_t *__restrict__ Bj,
+                                                      const int64_t *__restrict__ Bx, int64_t nrows,
+                                                      int lg, longlong2 *__restrict__ out)
+{
+    const int64_t total = nrows << lg;
+    const int64_t smask = ((int64_t) 1 << lg) - 1;
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t) gridDim.x * blockDim.x)
+    {
+        const int64_t k = t >> lg, sl = t & smask;
+        const int64_t p0 = Bp[k], len = Bp[k + 1] - p0;
+        long long c = 0, v = 0;
+        if (sl < len) { c = Bj[p0 + sl]; v = Bx[p0 + sl]; }
+        if (sl == 0) c |= (long long) ((unsigned long long) len << 56);
+        out[t] = make_longlong2(c, v);
+    }
+}
+
 constexpr int FUSED_NT = 256;              // threads of a fused-kernel workgroup
 constexpr int FUSED_ROWS = FUSED_NT / TG;  // rows of a tile
 constexpr unsigned long long LB_AGG = 1ull << 62;
@@ -747,7 +775,8 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 // CU (measured: 2% faster than four waves even with a few spilled
 // registers).  Positional multipliers keep four (their k array needs the
 // LDS); 64-bit keys double the key LDS, which then allows three.
-template <class S, class K, int NT, bool PK>
+// PK: 0 B read from Bp/Bj/Bx, 1 from the packed pairs, 2 from the row records
+template <class S, class K, int NT, int PK>
 __global__ __launch_bounds__(NT, (sizeof(K) == 4 ? (S::mult == GM_MULT_K ? 1024 : 1280) : 768) / NT)
 void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
@@ -800,8 +829,18 @@ void k_sx_fused(SxArgs g, int64_t fs,
                 if (e < a1)
                 {
                     k = g.Aj[e];
-                    bs = g.Bp[k];
-                    len = g.Bp[k + 1] - bs;
+                    if constexpr (PK == 2)
+                    {
+                        // the row's run of records; its length rides in the
+                        // first record, whose line the team reads next anyway
+                        bs = k << g.rec_lg;
+                        len = (int64_t) ((unsigned long long) g.Bjx[2 * bs] >> 56);
+                    }
+                    else
+                    {
+                        bs = g.Bp[k];
+                        len = g.Bp[k + 1] - bs;
+                    }
                     av = S::geta((const Z *) g.Ax, e, g.a_iso);
                 }
                 const int64_t inc = team_incl_scan<int64_t>(len);
@@ -828,10 +867,10 @@ void k_sx_fused(SxArgs g, int64_t fs,
                         const int64_t pb = s_bs[team][lo] + (q - s_pre[team][lo]);
                         int64_t j;
                         Z bv;
-                        if constexpr (PK)
+                        if constexpr (PK != 0)
                         {
                             const longlong2 jx = *(const longlong2 *) (g.Bjx + 2 * pb);
-                            j = jx.x;
+                            j = (PK == 2) ? (jx.x & SX_REC_COL) : jx.x;
                             bv = __builtin_bit_cast(Z, jx.y);
                         }
                         else
@@ -842,7 +881,7 @@ void k_sx_fused(SxArgs g, int64_t fs,
                             key[fq] = (K) (((K) j << 6) | (K) fq);
                             if constexpr (!S::iso_out)
                             {
-                                if constexpr (!PK) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
+                                if constexpr (PK == 0) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
                                 val[fq] = S::mul(s_av[team][lo], bv,
                                                  (S::mult == GM_MULT_K) ? s_k[KROWS > 1 ? team : 0][lo] : 0);
                             }
@@ -1160,6 +1199,7 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     g.gtab_off = goff.as<int64_t>(); g.gtab_log2 = glog.as<int32_t>();
     g.gkeys = nullptr; g.gvals = nullptr; g.glist = nullptr;
     g.Bjx = nullptr;
+    g.rec_lg = -1;
     const gm_saxpy3_params tu = saxpy3_tuned();
     g.ws_max = pick_param(prm ? prm->wave_small_max : 0, tu.wave_small_max, 64);
     g.wl_max = pick_param(prm ? prm->wave_large_max : 0, tu.wave_large_max, 512);
@@ -1294,19 +1334,30 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
             status.as<unsigned long long>(), ntiles + 1);
         k_sx_fill_u64<<<1, 64, 0, st>>>((unsigned long long *) g.cnt + 12, 1);
         GM_HIP(hipGetLastError(), "k_sx_fill_u64");
-        // B as 16-byte (column, value) pairs when its rows are gathered at
-        // least about once each on average (the pack reads and writes B once)
+        // B's rows: from its row records when the operand carries them (one
+        // aligned run per gathered row, no row pointers); else as 16-byte
+        // (column, value) pairs when its rows are gathered at least about once
+        // each on average (the pack reads and writes B once)
         DevBuf bjx;
+        int pk = 0;
         if constexpr (SX_PACK_OK<S>)
         {
-            if (!B->iso && B->x && nnzB > 0 && 2 * nnzA >= B->nrows &&
-                bjx.alloc(sizeof(longlong2) * nnzB))
+            const gm_csr_aux *ax = B->aux;
+            if (!B->iso && B->x && ax && ax->rec && ax->rec_lg >= 0 && ax->nrows == B->nrows)
+            {
+                g.Bjx = (const int64_t *) ax->rec;
+                g.rec_lg = ax->rec_lg;
+                pk = 2;
+            }
+            else if (!B->iso && B->x && nnzB > 0 && 2 * nnzA >= B->nrows &&
+                     bjx.alloc(sizeof(longlong2) * nnzB))
             {
                 ProfScope ps("sx_pack");
                 k_sx_pack<<<sx_grid(nnzB, 256, (int64_t) cu_count() * 16), 256, 0, st>>>(
                     B->j, (const int64_t *) B->x, nnzB, bjx.as<longlong2>());
                 GM_HIP(hipGetLastError(), "k_sx_pack");
                 g.Bjx = bjx.as<int64_t>();
+                pk = 1;
             }
         }
         {
@@ -1316,21 +1367,18 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
             unsigned long long *stp = status.as<unsigned long long>();
             int *errp = (int *) (g.cnt + 12);
             const bool k32 = B->ncols <= ((int64_t) 1 << 26);
+#define GM_SX_FUSED(K, PKV) \
+    k_sx_fused<S, K, FUSED_NT, PKV><<<(unsigned) grid, FUSED_NT, 0, st>>>(g, fs, stp, ntiles, errp)
             if constexpr (SX_PACK_OK<S>)
             {
-                if (g.Bjx && k32)
-                    k_sx_fused<S, uint32_t, FUSED_NT, true><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                        g, fs, stp, ntiles, errp);
-                else if (g.Bjx)
-                    k_sx_fused<S, uint64_t, FUSED_NT, true><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                        g, fs, stp, ntiles, errp);
+                if (pk == 2 && k32) GM_SX_FUSED(uint32_t, 2);
+                else if (pk == 2) GM_SX_FUSED(uint64_t, 2);
+                else if (pk == 1 && k32) GM_SX_FUSED(uint32_t, 1);
+                else if (pk == 1) GM_SX_FUSED(uint64_t, 1);
             }
-            if (!g.Bjx && k32)
-                k_sx_fused<S, uint32_t, FUSED_NT, false><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                    g, fs, stp, ntiles, errp);
-            else if (!g.Bjx)
-                k_sx_fused<S, uint64_t, FUSED_NT, false><<<(unsigned) grid, FUSED_NT, 0, st>>>(
-                    g, fs, stp, ntiles, errp);
+            if (pk == 0 && k32) GM_SX_FUSED(uint32_t, 0);
+            else if (pk == 0) GM_SX_FUSED(uint64_t, 0);
+#undef GM_SX_FUSED
             GM_HIP(hipGetLastError(), "k_sx_fused");
         }
         GM_HIP(hipMemcpyAsync(hp + 11, g.Cp + m, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(C)");
@@ -1491,4 +1539,64 @@ extern "C" int gm_saxpy3(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool ma
     GM_SEMIRING_LIST(GM_SX_CASE)
 #undef GM_SX_CASE
     return set_error(GM_NO_VALUE, "gm_saxpy3: semiring not instantiated");
+}
+
+//------------------------------------------------------------------------------
+// derived forms of an operand (gm_csr_aux)
+//------------------------------------------------------------------------------
+
+extern "C" int gm_csr_aux_build(gm_csr_aux **aux, const gm_csr *X)
+{
+    if (!aux || !X) return set_error(GM_INVALID_VALUE, "gm_csr_aux_build: NULL argument");
+    *aux = nullptr;
+    // row records need 8-byte values, one per entry, and columns below 2^56
+    if (!X->p || X->nrows <= 0 || !X->x || X->iso || type_size(X->type) != 8 ||
+        X->ncols >= ((int64_t) 1 << 56))
+        return GM_SUCCESS;
+    GM_TRY(ensure_init());
+    hipStream_t st = stream();
+    DevBuf cnt;
+    if (!cnt.alloc(8 * sizeof(unsigned long long))) return set_error(GM_OUT_OF_MEMORY, "aux: workspace");
+    unsigned long long *hp = (unsigned long long *) pinned_scratch(8 * sizeof(unsigned long long));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    k_sx_fill_u64<<<1, 64, 0, st>>>(cnt.as<unsigned long long>(), 8);
+    k_sx_maxlen<<<sx_grid(X->nrows, SX_NT * 4, (int64_t) cu_count() * 4), SX_NT, 0, st>>>(
+        X->p, X->nrows, X->p, 0, cnt.as<unsigned long long>());
+    GM_HIP(hipGetLastError(), "aux: k_sx_maxlen");
+    GM_HIP(hipMemcpyAsync(hp, cnt.ptr, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
+           "aux: lengths");
+    GM_HIP(hipStreamSynchronize(st), "aux: sync");
+    const int64_t maxlen = (int64_t) hp[0], nnz = (int64_t) hp[2];
+    int lg = 0;
+    while (((int64_t) 1 << lg) < maxlen) lg++;
+    if (nnz <= 0 || lg > SX_REC_MAX_LG || (X->nrows << lg) > nnz + nnz / 2 + 1024) return GM_SUCCESS;
+    gm_csr_aux *a = (gm_csr_aux *) calloc(1, sizeof(gm_csr_aux));
+    if (!a) return set_error(GM_OUT_OF_MEMORY, "aux: header");
+    a->nrows = X->nrows;
+    a->rec_lg = lg;
+    a->rec = gm_malloc(sizeof(longlong2) * (size_t) (X->nrows << lg));
+    if (!a->rec)
+    {
+        free(a);
+        return set_error(GM_OUT_OF_MEMORY, "aux: row records");
+    }
+    k_sx_rec_build<<<sx_grid(X->nrows << lg, 256, (int64_t) cu_count() * 16), 256, 0, st>>>(
+        X->p, X->j, (const int64_t *) X->x, X->nrows, lg, (longlong2 *) a->rec);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+    {
+        gm_free(a->rec);
+        free(a);
+        return check_hip(e, "aux: k_sx_rec_build");
+    }
+    *aux = a;
+    return GM_SUCCESS;
+}
+
+extern "C" void gm_csr_aux_free(gm_csr_aux *aux)
+{
+    if (!aux) return;
+    gm_free(aux->rec);
+    free(aux);
 }

@@ -166,7 +166,7 @@ static int sv_level(gm_server *s, std::vector<std::shared_ptr<SvRequest>> &batch
     }
     const int64_t n = s->graph.ncols;
     DevBuf tf, Pf, Jf, Xf, tv, Pv, Jv, Xv;
-    gm_csr Fs, Vs;
+    gm_csr Fs{}, Vs{};
     GM_TRY(bt_stack(pf, K, n, valued, zt, tf, Pf, Jf, Xf, &Fs));
     GM_TRY(bt_stack(pv, K, n, want_v, zt, tv, Pv, Jv, Xv, &Vs));
     if (!valued) Fs.x = nullptr;

@@ -64,7 +64,13 @@ struct GB_Descriptor_opaque
     bool builtin;
 };
 
-// CSR in HBM.  p == NULL means "no entries" (allocated on first use).
+// A matrix in HBM, held by row in one of three formats (sparsity):
+//   GxB_SPARSE  CSR in p/j/x; p == NULL means "no entries" (allocated on
+//               first use)
+//   GxB_BITMAP  b (one byte per entry position) and xd (nrows*ncols values)
+//   GxB_FULL    xd only, every entry present
+// For bitmap and full, p/j/x hold a CSR copy of the same content built on
+// first use by an operation that reads CSR (GB_matrix_csr), or are NULL.
 struct GB_Matrix_opaque
 {
     uint64_t magic;
@@ -73,6 +79,14 @@ struct GB_Matrix_opaque
     int64_t *p, *j;     // device
     void *x;            // device (one value if iso)
     bool iso;
+    int sparsity;           // GxB_SPARSE, GxB_BITMAP or GxB_FULL
+    int sparsity_control;   // GxB_SPARSITY_CONTROL hint (default GxB_AUTO_SPARSITY)
+    int8_t *b;              // device bitmap (GxB_BITMAP)
+    void *xd;               // device values by row (GxB_BITMAP, GxB_FULL; one if iso)
+    // derived forms of the current content (gm_csr_aux_build), built on first
+    // use by GrB_mxm and dropped with the content (GB_matrix_free_content)
+    gm_csr_aux *aux;
+    bool aux_tried;
     char err[384];
 };
 
@@ -85,6 +99,12 @@ GrB_Info GB_check_matrix(const GrB_Matrix A);
 GrB_Info GB_matrix_set_csr(GrB_Matrix A, const gm_csr_out *R);
 void GB_matrix_free_content(GrB_Matrix A);
 GrB_Info GB_matrix_view(gm_csr *v, GrB_Matrix A);
+GrB_Info GB_matrix_csr(GrB_Matrix A);
+void GB_matrix_dense_view(gm_dense *d, const GrB_Matrix A);
+GrB_Info GB_matrix_set_dense(GrB_Matrix A, const gm_dense_out *R);
+GrB_Info GB_matrix_conform(GrB_Matrix A);
+bool GB_matrix_is_dense(const GrB_Matrix A);
+void GB_matrix_aux(gm_csr *v, GrB_Matrix A);
 GrB_Type GB_type_from_code(int32_t code);
 
 // method selection (pure host logic, exported for tests)

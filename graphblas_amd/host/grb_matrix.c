@@ -6,6 +6,7 @@
 // device-resident arrays.
 
 #include "grb_internal.h"
+#include <stdarg.h>
 
 GrB_Info GB_check_matrix(const GrB_Matrix A)
 {
@@ -15,14 +16,28 @@ GrB_Info GB_check_matrix(const GrB_Matrix A)
     return GrB_SUCCESS;
 }
 
-void GB_matrix_free_content(GrB_Matrix A)
+// drops the CSR arrays (and their derived forms) only
+static void GB_matrix_free_csr(GrB_Matrix A)
 {
+    gm_csr_aux_free(A->aux);
+    A->aux = NULL;
+    A->aux_tried = false;
     gm_free(A->p);
     gm_free(A->j);
     gm_free(A->x);
     A->p = NULL;
     A->j = NULL;
     A->x = NULL;
+}
+
+void GB_matrix_free_content(GrB_Matrix A)
+{
+    GB_matrix_free_csr(A);
+    gm_free(A->b);
+    gm_free(A->xd);
+    A->b = NULL;
+    A->xd = NULL;
+    A->sparsity = GxB_SPARSE;
     A->nvals = 0;
     A->iso = false;
 }
@@ -38,10 +53,124 @@ GrB_Info GB_matrix_set_csr(GrB_Matrix A, const gm_csr_out *R)
     return GrB_SUCCESS;
 }
 
+bool GB_matrix_is_dense(const GrB_Matrix A)
+{
+    return A->sparsity == GxB_BITMAP || A->sparsity == GxB_FULL;
+}
+
+void GB_matrix_dense_view(gm_dense *d, const GrB_Matrix A)
+{
+    d->nrows = A->nrows;
+    d->ncols = A->ncols;
+    d->b = A->sparsity == GxB_BITMAP ? A->b : NULL;
+    d->x = A->xd;
+    d->iso = A->iso ? 1 : 0;
+    d->type = A->type->code;
+}
+
+// A takes a bitmap (R->b set) or full result; R's arrays move into A
+GrB_Info GB_matrix_set_dense(GrB_Matrix A, const gm_dense_out *R)
+{
+    GB_matrix_free_content(A);
+    A->b = R->b;
+    A->xd = R->x;
+    A->nvals = R->nvals;
+    A->iso = R->iso != 0;
+    A->sparsity = R->b ? GxB_BITMAP : GxB_FULL;
+    return GrB_SUCCESS;
+}
+
+// the CSR form of a bitmap or full matrix, built once per content
+GrB_Info GB_matrix_csr(GrB_Matrix A)
+{
+    if (!GB_matrix_is_dense(A) || A->p != NULL) return GrB_SUCCESS;
+    gm_dense d;
+    GB_matrix_dense_view(&d, A);
+    gm_csr_out R;
+    memset(&R, 0, sizeof(R));
+    int rc = gm_dense_to_csr(&R, &d);
+    if (rc != GM_SUCCESS) return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+    A->p = R.p;
+    A->j = R.j;
+    A->x = R.x;
+    return GrB_SUCCESS;
+}
+
+// Stores A in a format its GxB_SPARSITY_CONTROL allows (the reference's
+// GB_conform, Source/convert/GB_conform.c): a matrix with every entry present
+// goes full when full is allowed and its format is not; a format that is not
+// allowed moves to bitmap, else sparse; with only GxB_FULL allowed, a matrix
+// with entries missing is kept bitmap, as the reference does.
+GrB_Info GB_matrix_conform(GrB_Matrix A)
+{
+    int ctrl = A->sparsity_control & GxB_ANY_SPARSITY;
+    if (ctrl == 0) ctrl = GxB_AUTO_SPARSITY;
+    const bool sparse_ok = (ctrl & (GxB_SPARSE | GxB_HYPERSPARSE)) != 0;
+    const bool bitmap_ok = (ctrl & GxB_BITMAP) != 0;
+    const bool full_ok = (ctrl & GxB_FULL) != 0;
+    const bool all = A->nvals == A->nrows * A->ncols;
+    int want = A->sparsity;
+    if (A->sparsity == GxB_SPARSE && !sparse_ok)
+        want = (full_ok && all) ? GxB_FULL : GxB_BITMAP;
+    else if (A->sparsity == GxB_BITMAP && !bitmap_ok)
+        want = (full_ok && all) ? GxB_FULL : (sparse_ok ? GxB_SPARSE : GxB_BITMAP);
+    else if (A->sparsity == GxB_FULL && !full_ok)
+        want = bitmap_ok ? GxB_BITMAP : GxB_SPARSE;
+    if (want == A->sparsity) return GrB_SUCCESS;
+    GrB_Info info;
+    if (want == GxB_SPARSE)
+    {
+        if ((info = GB_matrix_csr(A)) != GrB_SUCCESS) return info;
+        gm_free(A->b);
+        gm_free(A->xd);
+        A->b = NULL;
+        A->xd = NULL;
+        A->sparsity = GxB_SPARSE;
+        return GrB_SUCCESS;
+    }
+    if (A->sparsity == GxB_FULL && want == GxB_BITMAP)
+    {
+        size_t mn = (size_t) (A->nrows * A->ncols);
+        A->b = (int8_t *) gm_malloc(mn > 0 ? mn : 1);
+        if (!A->b) return GB_error(A, GrB_OUT_OF_MEMORY, "conform: bitmap");
+        int rc = gm_memset(A->b, 1, mn);
+        if (rc == GM_SUCCESS) rc = gm_synchronize();
+        if (rc != GM_SUCCESS) return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+        A->sparsity = GxB_BITMAP;
+        return GrB_SUCCESS;
+    }
+    if (A->sparsity == GxB_BITMAP && want == GxB_FULL)
+    {
+        gm_free(A->b);
+        A->b = NULL;
+        A->sparsity = GxB_FULL;
+        return GrB_SUCCESS;
+    }
+    // sparse -> bitmap / full
+    gm_csr v;
+    if ((info = GB_matrix_view(&v, A)) != GrB_SUCCESS) return info;
+    gm_dense_out R;
+    memset(&R, 0, sizeof(R));
+    int rc = gm_csr_to_dense(&R, &v, want == GxB_FULL);
+    if (rc != GM_SUCCESS) return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+    const int64_t nv = A->nvals;
+    GB_matrix_free_csr(A);
+    A->b = R.b;
+    A->xd = R.x;
+    A->nvals = nv;
+    A->sparsity = want;
+    return GrB_SUCCESS;
+}
+
 // Device view of A for the kernel layer; an empty matrix gets zeroed row
 // pointers on first use.
 GrB_Info GB_matrix_view(gm_csr *v, GrB_Matrix A)
 {
+    if (GB_matrix_is_dense(A))
+    {
+        GrB_Info info = GB_matrix_csr(A);
+        if (info != GrB_SUCCESS) return info;
+    }
     if (A->p == NULL)
     {
         size_t bytes = sizeof(int64_t) * (size_t) (A->nrows + 1);
@@ -60,7 +189,21 @@ GrB_Info GB_matrix_view(gm_csr *v, GrB_Matrix A)
     v->x = A->x;
     v->iso = A->iso ? 1 : 0;
     v->type = A->type->code;
+    v->aux = A->aux;
     return GrB_SUCCESS;
+}
+
+// Builds A's derived forms once for its current content (a failure only
+// means none: the kernels then read p/j/x) and puts them in the view v.
+void GB_matrix_aux(gm_csr *v, GrB_Matrix A)
+{
+    if (!A->aux_tried)
+    {
+        A->aux_tried = true;
+        gm_csr_aux *aux = NULL;
+        if (gm_csr_aux_build(&aux, v) == GM_SUCCESS) A->aux = aux;
+    }
+    v->aux = A->aux;
 }
 
 GrB_Info GrB_Matrix_new(GrB_Matrix *A, GrB_Type type, GrB_Index nrows, GrB_Index ncols)
@@ -76,6 +219,8 @@ GrB_Info GrB_Matrix_new(GrB_Matrix *A, GrB_Type type, GrB_Index nrows, GrB_Index
     M->type = type;
     M->nrows = (int64_t) nrows;
     M->ncols = (int64_t) ncols;
+    M->sparsity = GxB_SPARSE;
+    M->sparsity_control = GxB_AUTO_SPARSITY;
     *A = M;
     return GrB_SUCCESS;
 }
@@ -155,6 +300,32 @@ GrB_Info GrB_Matrix_dup(GrB_Matrix *C, const GrB_Matrix A)
     info = GrB_Matrix_new(C, A->type, (GrB_Index) A->nrows, (GrB_Index) A->ncols);
     if (info != GrB_SUCCESS) return info;
     GrB_Matrix D = *C;
+    D->sparsity_control = A->sparsity_control;
+    if (GB_matrix_is_dense(A))
+    {
+        // bitmap / full: the by-row arrays
+        const size_t mn = (size_t) (A->nrows * A->ncols);
+        const size_t xb = A->type->size * (A->iso ? 1 : (mn > 0 ? mn : 1));
+        D->xd = gm_malloc(xb);
+        D->b = A->sparsity == GxB_BITMAP ? (int8_t *) gm_malloc(mn > 0 ? mn : 1) : NULL;
+        if (!D->xd || (A->sparsity == GxB_BITMAP && !D->b))
+        {
+            GrB_Matrix_free(C);
+            return GrB_OUT_OF_MEMORY;
+        }
+        int rc = gm_memcpy_d2d(D->xd, A->xd, xb);
+        if (rc == GM_SUCCESS && D->b) rc = gm_memcpy_d2d(D->b, A->b, mn);
+        if (rc == GM_SUCCESS) rc = gm_synchronize();
+        if (rc != GM_SUCCESS)
+        {
+            GrB_Matrix_free(C);
+            return GB_from_gm(rc);
+        }
+        D->sparsity = A->sparsity;
+        D->nvals = A->nvals;
+        D->iso = A->iso;
+        return GrB_SUCCESS;
+    }
     if (A->p == NULL) return GrB_SUCCESS;
     size_t pb = sizeof(int64_t) * (size_t) (A->nrows + 1);
     size_t jb = sizeof(int64_t) * (size_t) (A->nvals > 0 ? A->nvals : 1);
@@ -227,6 +398,8 @@ static GrB_Info GB_upload(GrB_Matrix A, const int64_t *Ap, const int64_t *Aj, co
 // download A into malloc'd host arrays
 static GrB_Info GB_download(GrB_Matrix A, int64_t **Ap, int64_t **Aj, void **Ax)
 {
+    GrB_Info info0 = GB_matrix_csr(A);
+    if (info0 != GrB_SUCCESS) return info0;
     int64_t n = A->nrows, nz = A->nvals;
     size_t xsz = A->type->size * (size_t) (A->iso ? 1 : (nz > 0 ? nz : 1));
     int64_t *hp = (int64_t *) malloc(sizeof(int64_t) * (size_t) (n + 1));
@@ -706,4 +879,236 @@ GrB_Info GxB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob,
 {
     (void) desc;
     return GrB_Matrix_deserialize(C, type, blob, blob_size);
+}
+
+//------------------------------------------------------------------------------
+// storage options (Include/GraphBLAS.h:5156-5160 of the reference)
+//------------------------------------------------------------------------------
+
+GrB_Info GxB_Matrix_Option_set_INT32(GrB_Matrix A, GxB_Option_Field field, int32_t value)
+{
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    switch (field)
+    {
+        case GxB_SPARSITY_CONTROL:
+            A->sparsity_control = (value & GxB_ANY_SPARSITY) ? (value & GxB_ANY_SPARSITY) : GxB_AUTO_SPARSITY;
+            return GB_matrix_conform(A);
+        case GxB_FORMAT:
+            if (value != GxB_BY_ROW)
+                return GB_error(A, GrB_NOT_IMPLEMENTED, "GxB_FORMAT: matrices are held by row only");
+            return GrB_SUCCESS;
+        default:
+            return GB_error(A, GrB_INVALID_VALUE, "option %d not settable as an int", (int) field);
+    }
+}
+
+GrB_Info GxB_Matrix_Option_get_INT32(GrB_Matrix A, GxB_Option_Field field, int32_t *value)
+{
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!value) return GrB_NULL_POINTER;
+    switch (field)
+    {
+        case GxB_SPARSITY_STATUS: *value = A->sparsity; return GrB_SUCCESS;
+        case GxB_SPARSITY_CONTROL: *value = A->sparsity_control; return GrB_SUCCESS;
+        case GxB_FORMAT: *value = GxB_BY_ROW; return GrB_SUCCESS;
+        default: return GB_error(A, GrB_INVALID_VALUE, "option %d has no int value", (int) field);
+    }
+}
+
+GrB_Info GxB_Matrix_Option_set(GrB_Matrix A, GxB_Option_Field field, ...)
+{
+    va_list ap;
+    va_start(ap, field);
+    GrB_Info info;
+    if (field == GxB_HYPER_SWITCH || field == GxB_BITMAP_SWITCH)
+    {
+        (void) va_arg(ap, double);          // no hypersparse format; the hint is accepted
+        info = GB_check_matrix(A);
+    }
+    else info = GxB_Matrix_Option_set_INT32(A, field, va_arg(ap, int));
+    va_end(ap);
+    return info;
+}
+
+GrB_Info GxB_Matrix_Option_get(GrB_Matrix A, GxB_Option_Field field, ...)
+{
+    va_list ap;
+    va_start(ap, field);
+    GrB_Info info;
+    if (field == GxB_HYPER_SWITCH || field == GxB_BITMAP_SWITCH)
+    {
+        double *d = va_arg(ap, double *);
+        info = GB_check_matrix(A);
+        if (info == GrB_SUCCESS && d) *d = 0;
+    }
+    else if (field == GxB_FORMAT)
+    {
+        GxB_Format_Value *f = va_arg(ap, GxB_Format_Value *);
+        info = GB_check_matrix(A);
+        if (info == GrB_SUCCESS && !f) info = GrB_NULL_POINTER;
+        if (info == GrB_SUCCESS) *f = GxB_BY_ROW;
+    }
+    else info = GxB_Matrix_Option_get_INT32(A, field, va_arg(ap, int32_t *));
+    va_end(ap);
+    return info;
+}
+
+//------------------------------------------------------------------------------
+// pack / unpack of bitmap and full matrices held by row
+//------------------------------------------------------------------------------
+
+// moves host arrays Ab (NULL: full) and Ax into A on the device
+static GrB_Info GB_pack_dense(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index Ab_size, GrB_Index Ax_size,
+                              bool iso, int64_t nvals)
+{
+    const int64_t mn = A->nrows * A->ncols;
+    const size_t xs = A->type->size;
+    if (!Ax || !*Ax) return GrB_NULL_POINTER;
+    if (Ab && !*Ab && mn > 0) return GrB_NULL_POINTER;
+    if (Ab && Ab_size < (GrB_Index) mn) return GB_error(A, GrB_INVALID_VALUE, "pack: Ab too small");
+    if (Ax_size < (GrB_Index) (xs * (size_t) (iso ? 1 : mn)))
+        return GB_error(A, GrB_INVALID_VALUE, "pack: Ax too small");
+    if (Ab)
+    {
+        int64_t cnt = 0;
+        for (int64_t t = 0; t < mn; t++) cnt += (*Ab)[t] != 0;
+        if (cnt != nvals) return GB_error(A, GrB_INVALID_VALUE, "pack_BitmapR: nvals does not match Ab");
+    }
+    const size_t xb = xs * (size_t) (iso ? 1 : (mn > 0 ? mn : 1));
+    int8_t *db = Ab ? (int8_t *) gm_malloc(mn > 0 ? (size_t) mn : 1) : NULL;
+    void *dx = gm_malloc(xb);
+    if (!dx || (Ab && !db))
+    {
+        gm_free(db);
+        gm_free(dx);
+        int rc = gm_init();
+        return GB_error(A, rc == GM_SUCCESS ? GrB_OUT_OF_MEMORY : GrB_PANIC, "%s", gm_last_error());
+    }
+    int rc = GM_SUCCESS;
+    if (db && mn > 0) rc = gm_memcpy_h2d(db, *Ab, (size_t) mn);
+    if (rc == GM_SUCCESS && (mn > 0 || iso)) rc = gm_memcpy_h2d(dx, *Ax, xs * (size_t) (iso ? 1 : mn));
+    if (rc != GM_SUCCESS)
+    {
+        gm_free(db);
+        gm_free(dx);
+        return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+    }
+    GB_matrix_free_content(A);
+    A->b = db;
+    A->xd = dx;
+    A->nvals = nvals;
+    A->iso = iso;
+    A->sparsity = Ab ? GxB_BITMAP : GxB_FULL;
+    if (Ab)
+    {
+        free(*Ab);
+        *Ab = NULL;
+    }
+    free(*Ax);
+    *Ax = NULL;
+    return GB_matrix_conform(A);
+}
+
+GrB_Info GxB_Matrix_pack_BitmapR(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index Ab_size,
+                                 GrB_Index Ax_size, bool iso, GrB_Index nvals, const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!Ab) return GrB_NULL_POINTER;
+    return GB_pack_dense(A, Ab, Ax, Ab_size, Ax_size, iso, (int64_t) nvals);
+}
+
+GrB_Info GxB_Matrix_pack_FullR(GrB_Matrix A, void **Ax, GrB_Index Ax_size, bool iso,
+                               const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    return GB_pack_dense(A, NULL, Ax, 0, Ax_size, iso, A->nrows * A->ncols);
+}
+
+// moves A out as malloc'd host arrays in bitmap (Ab != NULL) or full form
+static GrB_Info GB_unpack_dense(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index *Ab_size,
+                                GrB_Index *Ax_size, bool *iso, GrB_Index *nvals)
+{
+    const int64_t mn = A->nrows * A->ncols;
+    const bool full = (Ab == NULL);
+    if (full && A->nvals != mn) return GB_error(A, GrB_INVALID_VALUE, "unpack_FullR: entries are missing");
+    gm_dense_out R;
+    memset(&R, 0, sizeof(R));
+    const int8_t *sb;
+    const void *sx;
+    if (GB_matrix_is_dense(A))
+    {
+        sb = A->sparsity == GxB_BITMAP ? A->b : NULL;
+        sx = A->xd;
+    }
+    else
+    {
+        gm_csr v;
+        GrB_Info info = GB_matrix_view(&v, A);
+        if (info != GrB_SUCCESS) return info;
+        int rc = gm_csr_to_dense(&R, &v, full);
+        if (rc != GM_SUCCESS) return GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+        sb = R.b;
+        sx = R.x;
+    }
+    const size_t xs = A->type->size;
+    const size_t xb = xs * (size_t) (A->iso ? 1 : (mn > 0 ? mn : 1));
+    int8_t *hb = full ? NULL : (int8_t *) malloc(mn > 0 ? (size_t) mn : 1);
+    void *hx = malloc(xb);
+    GrB_Info info = GrB_SUCCESS;
+    if (!hx || (!full && !hb)) info = GrB_OUT_OF_MEMORY;
+    int rc = GM_SUCCESS;
+    if (info == GrB_SUCCESS && !full && mn > 0)
+    {
+        if (sb) rc = gm_memcpy_d2h(hb, sb, (size_t) mn);
+        else memset(hb, 1, (size_t) mn);            // full: every entry present
+    }
+    if (info == GrB_SUCCESS && rc == GM_SUCCESS && (mn > 0 || A->iso))
+        rc = gm_memcpy_d2h(hx, sx, xs * (size_t) (A->iso ? 1 : mn));
+    gm_free(R.b);
+    gm_free(R.x);
+    if (info == GrB_SUCCESS && rc != GM_SUCCESS) info = GB_error(A, GB_from_gm(rc), "%s", gm_last_error());
+    if (info != GrB_SUCCESS)
+    {
+        free(hb);
+        free(hx);
+        return info;
+    }
+    if (!full)
+    {
+        *Ab = hb;
+        *Ab_size = (GrB_Index) (mn > 0 ? mn : 1);
+    }
+    *Ax = hx;
+    *Ax_size = (GrB_Index) xb;
+    if (iso) *iso = A->iso;
+    if (nvals) *nvals = (GrB_Index) A->nvals;
+    GB_matrix_free_content(A);
+    return GrB_SUCCESS;
+}
+
+GrB_Info GxB_Matrix_unpack_BitmapR(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index *Ab_size,
+                                   GrB_Index *Ax_size, bool *iso, GrB_Index *nvals,
+                                   const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!Ab || !Ax || !Ab_size || !Ax_size) return GrB_NULL_POINTER;
+    return GB_unpack_dense(A, Ab, Ax, Ab_size, Ax_size, iso, nvals);
+}
+
+GrB_Info GxB_Matrix_unpack_FullR(GrB_Matrix A, void **Ax, GrB_Index *Ax_size, bool *iso,
+                                 const GrB_Descriptor desc)
+{
+    (void) desc;
+    GrB_Info info = GB_check_matrix(A);
+    if (info != GrB_SUCCESS) return info;
+    if (!Ax || !Ax_size) return GrB_NULL_POINTER;
+    return GB_unpack_dense(A, NULL, Ax, NULL, Ax_size, iso, NULL);
 }

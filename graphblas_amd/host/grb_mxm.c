@@ -74,6 +74,7 @@ bool GB_AxB_dot2_control(double anz, double bnz, double anvec, double bnvec, dou
 static gm_csr GB_view_of(const gm_csr_out *R)
 {
     gm_csr v;
+    memset(&v, 0, sizeof(v));
     v.nrows = R->nrows;
     v.ncols = R->ncols;
     v.p = R->p;
@@ -170,6 +171,75 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
     }
     // T reaches C through GB_accum_mask when C's old entries matter
     const bool post = C->nvals > 0 && (accum != NULL || (Mask != NULL && !C_replace));
+
+    // bitmap / full operands without transposes or a method hint
+    // (GB_AxB_saxpy.c:89-140, GB_AxB_saxpy_sparsity.c:177-234): C += A*B
+    // into a full C with the monoid as accum (saxpy4 / saxpy5), or a bitmap C
+    // (saxbit) when the sparsity rule gives one
+    if ((GB_matrix_is_dense(A) || GB_matrix_is_dense(B)) && !a_tran && !b_tran && axb == GxB_DEFAULT)
+    {
+        const bool ad = GB_matrix_is_dense(A), bd = GB_matrix_is_dense(B);
+        const GrB_BinaryOp mon = semiring->add->op;
+        const bool in_place = C->sparsity == GxB_FULL && !C->iso && !Mask && accum != NULL &&
+                              accum->opcode == mon->opcode && accum->ztype == mon->ztype &&
+                              ad != bd && sr.add_op != GM_ADD_ANY;
+        bool bitmap = false;
+        if (!in_place && !post && (!Mask || mask_comp || GB_matrix_is_dense(Mask)))
+        {
+            if (ad && bd) bitmap = true;
+            else if (bd)
+            {
+                // {S} * {B,F}: bitmap unless A has few nonempty rows
+                gm_csr va0;
+                if ((info = GB_matrix_view(&va0, A)) != GrB_SUCCESS) return GB_error(C, info, "%s", A->err);
+                int64_t anvec = 0;
+                int rc0 = gm_nonempty_rows(&anvec, &va0);
+                if (rc0 != GM_SUCCESS) return GB_error(C, GB_from_gm(rc0), "GrB_mxm: %s", gm_last_error());
+                bitmap = anvec >= C->nrows / 4;
+            }
+            else bitmap = !(B->nvals < C->ncols / 20);   // {B,F} * {S}: sparse if B is very sparse
+        }
+        if (in_place || bitmap)
+        {
+            gm_csr as, bs, ms;
+            gm_dense adn, bdn, mdn, cdn;
+            const gm_csr *pas = NULL, *pbs = NULL, *pms = NULL;
+            const gm_dense *pad = NULL, *pbd = NULL, *pmd = NULL;
+            if (ad) { GB_matrix_dense_view(&adn, A); pad = &adn; }
+            else
+            {
+                if ((info = GB_matrix_view(&as, A)) != GrB_SUCCESS) return GB_error(C, info, "%s", A->err);
+                pas = &as;
+            }
+            if (bd) { GB_matrix_dense_view(&bdn, B); pbd = &bdn; }
+            else
+            {
+                if ((info = GB_matrix_view(&bs, B)) != GrB_SUCCESS) return GB_error(C, info, "%s", B->err);
+                pbs = &bs;
+            }
+            if (Mask && GB_matrix_is_dense(Mask)) { GB_matrix_dense_view(&mdn, Mask); pmd = &mdn; }
+            else if (Mask)
+            {
+                if ((info = GB_matrix_view(&ms, Mask)) != GrB_SUCCESS)
+                    return GB_error(C, info, "%s", Mask->err);
+                pms = &ms;
+            }
+            const bool mstruct0 = mask_struct || (Mask && !GB_matrix_is_dense(Mask) && Mask->x == NULL);
+            if (in_place) GB_matrix_dense_view(&cdn, C);
+            if (GB_burble)
+                printf("burble: GrB_mxm: %s (A %s, B %s)\n",
+                       in_place ? (ad ? "C+=A*B saxpy5" : "C+=A*B saxpy4") : "C bitmap saxbit",
+                       ad ? "bitmap/full" : "sparse", bd ? "bitmap/full" : "sparse");
+            gm_dense_out D;
+            memset(&D, 0, sizeof(D));
+            int rc1 = gm_saxbit(&D, pms, pmd, mask_comp, mstruct0, pas, pad, pbs, pbd, &sr,
+                                in_place ? &cdn : NULL);
+            if (rc1 != GM_SUCCESS) return GB_error(C, GB_from_gm(rc1), "GrB_mxm: %s", gm_last_error());
+            GB_matrix_set_dense(C, &D);
+            C->err[0] = '\0';
+            return GB_matrix_conform(C);
+        }
+    }
 
     GB_method method = GB_mxm_select(Mask != NULL, mask_comp, a_tran, b_tran, axb);
 
@@ -289,6 +359,11 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
                    Mask ? (mask_comp ? "<!M>" : "<M>") : "", GB_sr_name(&sr, nm, sizeof(nm)),
                    (long long) (a_tran ? AT.nvals : A->nvals),
                    (long long) (b_tran ? BT.nvals : B->nvals));
+        // B's row records (built once per content of B) when the fused kernel
+        // gathers its 8-byte values row by row, about once per row or more
+        if (!b_tran && !B->iso && B->x && B->type->size == 8 && sr.mult_op != GM_MULT_PAIR &&
+            sr.mult_op != GM_MULT_K && 2 * (a_tran ? AT.nvals : A->nvals) >= B->nrows)
+            GB_matrix_aux(&vb, B);
         rc = gm_saxpy3(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, NULL);
     }
     GB_free_out(&AT);
@@ -315,8 +390,8 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         if (rc != GM_SUCCESS) return GB_error(C, GB_from_gm(rc), "GrB_mxm: %s", gm_last_error());
         R = O;
     }
-    // transplant T into C (GB_mxm.c:168-200)
+    // transplant T into C (GB_mxm.c:168-200), then C's format hint
     GB_matrix_set_csr(C, &R);
     C->err[0] = '\0';
-    return GrB_SUCCESS;
+    return GB_matrix_conform(C);
 }

@@ -25,6 +25,7 @@ static GrB_Info GB_reduce(void *c, int32_t ccode, const GrB_BinaryOp accum,
         case GB_OP_ANY: add = GM_ADD_ANY; break;
         default: return GrB_NOT_IMPLEMENTED;
     }
+    if ((info = GB_matrix_csr(A)) != GrB_SUCCESS) return info;   // bitmap / full: its CSR form
     union { int64_t i64; int32_t i32; double f64; uint8_t b; } r;
     memset(&r, 0, sizeof(r));
     int rc = gm_reduce_scalar(&r, add, mt, A->x, A->p ? A->nvals : 0, A->iso ? 1 : 0);

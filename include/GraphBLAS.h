@@ -310,6 +310,46 @@ GrB_Info GrB_Vector_error(const char **error, const GrB_Vector v);
     _Generic((object), GrB_Vector: GrB_Vector_error, default: GrB_Matrix_error)(error, object)
 #endif
 
+//------------------------------------------------------------------------------
+// storage formats: sparse (CSR), bitmap and full, all held by row
+// (the reference's GxB_Option_Field values and sparsity bits,
+// Include/GraphBLAS.h:1389-1495; pack/unpack at :9686-9775, :10110-10167)
+//------------------------------------------------------------------------------
+
+typedef enum
+{
+    GxB_HYPER_SWITCH = 7000,        // accepted and ignored (no hypersparse format)
+    GxB_BITMAP_SWITCH = 7001,       // accepted and ignored
+    GxB_FORMAT = 7002,              // GxB_BY_ROW only
+    GxB_SPARSITY_STATUS = 7034,     // get: GxB_SPARSE, GxB_BITMAP or GxB_FULL
+    GxB_SPARSITY_CONTROL = 7036     // set/get: allowed formats (sum of the bits below)
+} GxB_Option_Field;
+
+typedef enum { GxB_BY_ROW = 0, GxB_BY_COL = 1, GxB_NO_FORMAT = -1 } GxB_Format_Value;
+
+#define GxB_HYPERSPARSE 1   // treated as sparse
+#define GxB_SPARSE      2
+#define GxB_BITMAP      4
+#define GxB_FULL        8
+#define GxB_ANY_SPARSITY (GxB_HYPERSPARSE + GxB_SPARSE + GxB_BITMAP + GxB_FULL)
+#define GxB_AUTO_SPARSITY GxB_ANY_SPARSITY
+
+GrB_Info GxB_Matrix_Option_set(GrB_Matrix A, GxB_Option_Field field, ...);
+GrB_Info GxB_Matrix_Option_get(GrB_Matrix A, GxB_Option_Field field, ...);
+GrB_Info GxB_Matrix_Option_set_INT32(GrB_Matrix A, GxB_Option_Field field, int32_t value);
+GrB_Info GxB_Matrix_Option_get_INT32(GrB_Matrix A, GxB_Option_Field field, int32_t *value);
+
+GrB_Info GxB_Matrix_pack_BitmapR(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index Ab_size,
+                                 GrB_Index Ax_size, bool iso, GrB_Index nvals,
+                                 const GrB_Descriptor desc);
+GrB_Info GxB_Matrix_pack_FullR(GrB_Matrix A, void **Ax, GrB_Index Ax_size, bool iso,
+                               const GrB_Descriptor desc);
+GrB_Info GxB_Matrix_unpack_BitmapR(GrB_Matrix A, int8_t **Ab, void **Ax, GrB_Index *Ab_size,
+                                   GrB_Index *Ax_size, bool *iso, GrB_Index *nvals,
+                                   const GrB_Descriptor desc);
+GrB_Info GxB_Matrix_unpack_FullR(GrB_Matrix A, void **Ax, GrB_Index *Ax_size, bool *iso,
+                                 const GrB_Descriptor desc);
+
 #define GxB_BURBLE 7019   // GxB_Option_Field value in the reference
 GrB_Info GxB_Global_Option_set_INT32(int field, int32_t value);
 GrB_Info GxB_Global_Option_get_INT32(int field, int32_t *value);

@@ -81,6 +81,11 @@ typedef struct
     int32_t ztype;      // gm_type of the result (and of x,y for valued ops)
 } gm_semiring;
 
+// Derived forms of a CSR operand (opaque; gm_csr_aux_build below).  Built
+// once and kept with the operand while its content is unchanged, as the
+// reference keeps a matrix's hyper-hash A->Y (Source/builtin/include/GB_matrix.h:243-306).
+typedef struct gm_csr_aux gm_csr_aux;
+
 // read-only CSR operand in device memory; rows sorted, no duplicates
 typedef struct
 {
@@ -90,6 +95,8 @@ typedef struct
     const void *x;      // values (NULL: pattern only / all true)
     int32_t iso;        // 1: x holds one value used for every entry
     int32_t type;       // gm_type of x
+    const gm_csr_aux *aux;  // NULL, or derived forms of this same content that
+                            // kernels may read in place of p/j/x
 } gm_csr;
 
 // CSR result; p/j/x are allocated by the library with gm_malloc and owned by
@@ -104,6 +111,66 @@ typedef struct
     int32_t iso;
     int32_t type;
 } gm_csr_out;
+
+// Bitmap or full matrix held by rows in device memory: entry (i,j) at
+// i*ncols + j; b[i*ncols + j] != 0 marks a present entry, b == NULL means
+// full (every entry present); x holds nrows*ncols values (one if iso).  The
+// reference's GxB_BITMAP / GxB_FULL formats (Include/GraphBLAS.h:1491-1495)
+// by row (GxB_Matrix_pack_BitmapR / pack_FullR, :9686-9775).
+typedef struct
+{
+    int64_t nrows, ncols;
+    const int8_t *b;
+    const void *x;
+    int32_t iso;
+    int32_t type;
+} gm_dense;
+
+// bitmap/full result; b and x allocated with gm_malloc and owned by the caller
+typedef struct
+{
+    int64_t nrows, ncols;
+    int64_t nvals;      // present entries
+    int8_t *b;          // NULL: full
+    void *x;
+    int32_t iso;
+    int32_t type;
+} gm_dense_out;
+
+// format conversions on the device: the present entries of D as sorted CSR;
+// A as bitmap (full = false) or full (full = true: every entry must be
+// present, else GM_INVALID_VALUE)
+int gm_dense_to_csr(gm_csr_out *R, const gm_dense *D);
+int gm_csr_to_dense(gm_dense_out *D, const gm_csr *A, bool full);
+
+// Products with a bitmap or full result (the reference's dense-result saxpy
+// methods): C = A*B, C<M> = A*B or C<!M> = A*B with C bitmap (GB_AxB_saxbit,
+// Source/mxm/GB_AxB_saxbit.c), or, when Cin is given, C = Cin (+) A*B with C
+// full and the semiring's monoid as the accumulator (GB_AxB_saxpy4 for A
+// sparse and B bitmap/full, GB_AxB_saxpy5 for A bitmap/full and B sparse,
+// Source/mxm/GB_AxB_saxpy4.c, GB_AxB_saxpy5.c; chosen at
+// Source/mxm/GB_AxB_saxpy.c:89-140).  Every operand is either sparse (the
+// gm_csr) or bitmap/full (the gm_dense): exactly one of As/Ad and of Bs/Bd
+// is given, at most one of Ms/Md (none: no mask).  Terms are applied for k
+// ascending, so fp64 sums equal the reference's sequential sums; ANY keeps
+// the last writer.  Returns GM_NO_VALUE for a case it does not handle
+// (Cin with a mask, an ANY monoid, an iso or bitmap Cin, mixed types).
+int gm_saxbit(gm_dense_out *C, const gm_csr *Ms, const gm_dense *Md, bool mask_comp, bool mask_struct,
+              const gm_csr *As, const gm_dense *Ad, const gm_csr *Bs, const gm_dense *Bd,
+              const gm_semiring *sr, const gm_dense *Cin);
+
+// Builds the derived forms of X that apply to it (*aux = NULL when none does):
+//   row records: every row of at most 16 entries with 8-byte values stored as
+//   a fixed-stride run of 16-byte (column, value) records (stride: a power of
+//   two >= the longest row; built only while the padding stays under half of
+//   nnz), the row's length in the top byte of its first record's column.  A
+//   saxpy3 gather of row k then reads one aligned run at k*stride and needs
+//   neither the row pointers nor a second cache line.
+// The forms describe X's content at the time of the call: free them (and
+// build again) when the content changes.  Synchronises the stream, so the
+// forms can be used from any thread's stream afterwards.
+int gm_csr_aux_build(gm_csr_aux **aux, const gm_csr *X);
+void gm_csr_aux_free(gm_csr_aux *aux);
 
 // dot3 tuning knobs (0 = default).  Work per mask entry is
 // min(|A(i,:)|+|B(j,:)|, s*ceil(log2(l+1))) with s,l the short/long list.
