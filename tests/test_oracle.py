@@ -2,9 +2,12 @@
 
 The reference holds no golden vectors for this path (its tests regenerate
 random inputs and compare against a dense brute force, Test/GB_spec_mxm.m:
-99-114, or MATLAB's A*B at 1e-12, Test/test246.m), and its library cannot be
-built here without its CMake configure step (generated Config/GB_config.h).
-So each oracle function is checked against what the mathematics fixes:
+99-114, or MATLAB's A*B at 1e-12, Test/test246.m).  Its library does build
+from a scratch copy with its own CMake configure step (which generates
+Config/GB_config.h and compiles hundreds of files), but this work does not
+run the reference's build system or write stand-ins for its generated
+headers, so the reference library is not available as a checker here.  Each
+oracle function is checked instead against what the mathematics fixes:
 
 * dense brute force on small non-square inputs (a transposed operand, a
   dropped term or a wrong index changes the result),
