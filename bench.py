@@ -218,6 +218,29 @@ def bench_served(args, capi, W):
         return float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
     p50, p99 = pct(r_po)
     q50, q99 = pct(r_pp)
+    # roofline of the served kernels: the saxpy3 single-pass gather model of
+    # every request (F's rows and entries, 16 B of graph row pointers and
+    # 8 B per neighbour of every source, C written as index + int64 value)
+    # over the kernel time of one profiled saturation pass
+    nnzc = np.array([r.nvals for r in r_srv], dtype=np.int64)
+    ks = np.array([len(s) for s in reqs], dtype=np.int64)
+    srv_bytes = int((8 * (ks + 1) + 8 * ks + 16 * ks + 8 * flops + 8 * (ks + 1) + 16 * nnzc).sum())
+    capi.prof_reset()
+    capi.prof_enable(True)
+    run_server(capi, W, dG.view, reqs, 0, 2, 256, budget)
+    capi.lib().gm_synchronize()
+    capi.prof_enable(False)
+    kern = capi.prof_get()
+    ktot = sum(v[1] for v in kern.values())
+    roof = None
+    if ktot > 0:
+        ach = srv_bytes / (ktot * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "all kernels of a saturation pass (summed event times; batches "
+                          "overlap on the device, so this understates the achieved rate)",
+                "kernel_ms": ktot, "kernel_alg_bytes": srv_bytes,
+                "kernels_ms": {k: v[1] for k, v in sorted(kern.items())}}
     out = {"metric": "served frontier expansion: semiring mults/s (saturation) with p50/p99 ms "
                      "at a fixed Poisson rate",
            "unit": "mults/s", "dtype": "int64",
@@ -239,7 +262,7 @@ def bench_served(args, capi, W):
                        "plain_p50_ms": q50, "plain_p99_ms": q99,
                        "plain_achieved_per_s": len(reqs) / t_pp},
            "total_mults": total_flops,
-           "roofline": None}
+           "roofline": roof}
     if not args.no_cpu:
         # oracle on a stacked sample of the requests (C<!F> = F*A)
         nr = max(1, len(reqs) // 8)
@@ -326,8 +349,10 @@ def bench_batched(args, capi, W):
     ref = plain(True)
     got = batched(True)
     same = all(ref[g] == got[g] for g in range(nreq))
-    # total semiring multiplies: common neighbours over all mask entries
+    # total semiring multiplies: common neighbours over all mask entries;
+    # algorithmic bytes of every request (probe model, as dot3_alg_bytes)
     mults = 0
+    alg_bytes = 0
     for g in range(nreq):
         out = capi.GmCsrOut()
         sr = capi.GmSemiring(1, 3, 8)
@@ -339,6 +364,16 @@ def bench_batched(args, capi, W):
         r = ctypes.c_int64()
         L.gm_reduce_scalar(ctypes.byref(r), 1, 8, out.x, out.nvals, out.iso)
         mults += r.value
+        mn, ln = ctypes.c_int64(), ctypes.c_int64()
+        if L.gm_dot3_work(ctypes.byref(mn), ctypes.byref(ln), ctypes.byref(v), ctypes.byref(v),
+                          ctypes.byref(v)) != 0:
+            raise RuntimeError(capi.last_error())
+        nm = int(db.nzbase[g + 1] - db.nzbase[g])
+        sab = 4 if specs[g].semiring == "min_plus_int32" else 0
+        # M: row pointers, indices, int32 mask values; C: row pointers,
+        # indices, int32 values; per entry 16 B of row pointers + the shorter list
+        alg_bytes += (8 * (specs[g].n + 1) + 12 * nm + 8 * (specs[g].n + 1) + 12 * out.nvals
+                      + 16 * nm + (8 + sab) * mn.value)
         for ptr in (out.p, out.j, out.x):
             L.gm_free(ptr)
     # both legs timed the same way, without the per-kernel events; the
@@ -368,6 +403,8 @@ def bench_batched(args, capi, W):
     kern = capi.prof_get()
     steps_prof = 1
     nnz = int(db.nzbase[-1])
+    ktot = sum(v[1] for v in kern.values()) / steps_prof
+    ach = alg_bytes / (ktot * 1e-3) / 1e9 if ktot > 0 else 0.0
     ns = np.array([s.n for s in specs])
     out = {"metric": "batched uneven mxm: semiring mults/s, batched vs plain one-call-per-request",
            "unit": "mults/s", "dtype": "int32",
@@ -388,7 +425,10 @@ def bench_batched(args, capi, W):
            "generate_ms": t_gen * 1e3,
            "tuning": os.path.relpath(tuning, ROOT) if tuning else None,
            "kernels_ms_per_step": {k: v[1] / steps_prof for k, v in sorted(kern.items())},
-           "roofline": None}
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": "all kernels of one batched step (summed event times)",
+                        "kernel_ms": ktot, "kernel_alg_bytes": int(alg_bytes)}}
     if not args.no_cpu:
         import pyoracle as O
         hosts, units = [], 0

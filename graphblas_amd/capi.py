@@ -258,6 +258,8 @@ def lib():
         "gm_reduce_scalar_noalloc": ([P, I32, I32, P, I64, I32, P], ctypes.c_int),
         "gm_csr_aux_build": ([PP, ctypes.POINTER(GmCsr)], ctypes.c_int),
         "gm_dense_to_csr": ([ctypes.POINTER(GmCsrOut), ctypes.POINTER(GmDense)], ctypes.c_int),
+        "gm_dot3_work": ([ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(GmCsr),
+                          ctypes.POINTER(GmCsr), ctypes.POINTER(GmCsr)], ctypes.c_int),
         "gm_csr_to_dense": ([ctypes.POINTER(GmDenseOut), ctypes.POINTER(GmCsr), ctypes.c_bool],
                             ctypes.c_int),
         "gm_saxbit": ([ctypes.POINTER(GmDenseOut), ctypes.POINTER(GmCsr), ctypes.POINTER(GmDense),

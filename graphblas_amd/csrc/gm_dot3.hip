@@ -2464,6 +2464,65 @@ extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const g
     return rc;
 }
 
+// sums over the mask entries (i,j) of min(|A(i,:)|, |B(j,:)|) and of
+// |A(i,:)| + |B(j,:)|: the list lengths the probe and merge models of a dot3
+// call read (one thread per mask row, a partial per workgroup)
+__global__ __launch_bounds__(D3_NT) void k_d3_work(const int64_t *__restrict__ Mp,
+                                                   const int64_t *__restrict__ Mj, int64_t n,
+                                                   const int64_t *__restrict__ Ap,
+                                                   const int64_t *__restrict__ Bp,
+                                                   unsigned long long *__restrict__ part)
+{
+    __shared__ unsigned long long s[D3_NT / 64 + 1];
+    unsigned long long smin = 0, ssum = 0;
+    for (int64_t i = (int64_t) blockIdx.x * D3_NT + threadIdx.x; i < n; i += (int64_t) gridDim.x * D3_NT)
+    {
+        const int64_t a = Ap[i + 1] - Ap[i];
+        for (int64_t p = Mp[i]; p < Mp[i + 1]; p++)
+        {
+            const int64_t j = Mj[p];
+            const int64_t b = Bp[j + 1] - Bp[j];
+            smin += (unsigned long long) (a < b ? a : b);
+            ssum += (unsigned long long) (a + b);
+        }
+    }
+    const unsigned long long tmin = block_sum<unsigned long long, D3_NT>(smin, s);
+    const unsigned long long tsum = block_sum<unsigned long long, D3_NT>(ssum, s);
+    if (threadIdx.x == 0)
+    {
+        atomicAdd(&part[0], tmin);
+        atomicAdd(&part[1], tsum);
+    }
+}
+
+extern "C" int gm_dot3_work(int64_t *min_sum, int64_t *len_sum, const gm_csr *M, const gm_csr *A,
+                            const gm_csr *B)
+{
+    if (!min_sum || !len_sum || !M || !A || !B)
+        return set_error(GM_INVALID_VALUE, "gm_dot3_work: NULL argument");
+    if (M->nrows != A->nrows || M->ncols != B->nrows)
+        return set_error(GM_INVALID_VALUE, "gm_dot3_work: M must be nrows(A) x nrows(B)");
+    *min_sum = 0;
+    *len_sum = 0;
+    if (M->nrows == 0) return GM_SUCCESS;
+    if (!M->p || !A->p || !B->p) return set_error(GM_INVALID_VALUE, "gm_dot3_work: NULL row pointers");
+    GM_TRY(ensure_init());
+    hipStream_t st = stream();
+    DevBuf part;
+    if (!part.alloc(2 * sizeof(unsigned long long))) return set_error(GM_OUT_OF_MEMORY, "gm_dot3_work");
+    k_zero_u64<<<1, 64, 0, st>>>(part.as<unsigned long long>(), 2);
+    k_d3_work<<<grid_for(M->nrows, D3_NT, (int64_t) cu_count() * 8), D3_NT, 0, st>>>(
+        M->p, M->j, M->nrows, A->p, B->p, part.as<unsigned long long>());
+    GM_HIP(hipGetLastError(), "k_d3_work");
+    unsigned long long *hp = (unsigned long long *) pinned_scratch(2 * sizeof(unsigned long long));
+    if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+    GM_HIP(hipMemcpyAsync(hp, part.ptr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st), "work");
+    GM_HIP(hipStreamSynchronize(st), "gm_dot3_work sync");
+    *min_sum = (int64_t) hp[0];
+    *len_sum = (int64_t) hp[1];
+    return GM_SUCCESS;
+}
+
 extern "C" int gm_nonempty_rows(int64_t *count, const gm_csr *A)
 {
     if (!count || !A) return set_error(GM_INVALID_VALUE, "gm_nonempty_rows: NULL argument");

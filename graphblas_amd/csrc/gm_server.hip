@@ -77,6 +77,12 @@ struct SvRequest
     std::vector<int64_t> src;
     int64_t k = 0;
     SvSlice F, V, C;                  // frontier, visited set, last level's result
+    // before the first level F and V are the host arrays below (F: one entry
+    // per source; V: the visited rows with their values), uploaded with the
+    // rest of the batch that first runs the request; later levels read the
+    // HBM slices above
+    std::vector<int64_t> hvp, hvj;
+    std::vector<uint8_t> hvx;
     int64_t flops = 0;                // flop bound of the next level
     int64_t flops_total = 0;          // flop bounds of the levels run
     int64_t max_steps = 1;            // levels to run (<= 0: until the frontier is empty)
@@ -147,28 +153,103 @@ struct gm_server
     double flops_done = 0;
 };
 
+// value v of type zt as bytes
+static void sv_value(uint8_t *out, int32_t zt, int64_t v)
+{
+    switch (zt)
+    {
+        case GM_FP64: { double d = (double) v; memcpy(out, &d, 8); break; }
+        case GM_FP32: { float f = (float) v; memcpy(out, &f, 4); break; }
+        case GM_INT64: case GM_UINT64: memcpy(out, &v, 8); break;
+        case GM_INT32: case GM_UINT32: { int32_t w = (int32_t) v; memcpy(out, &w, 4); break; }
+        case GM_INT16: case GM_UINT16: { int16_t w = (int16_t) v; memcpy(out, &w, 2); break; }
+        default: out[0] = (uint8_t) (v != 0);
+    }
+}
+
 // one level of every request in the batch; returns the status of the batch
 static int sv_level(gm_server *s, std::vector<std::shared_ptr<SvRequest>> &batch)
 {
     const int32_t zt = s->sr.ztype;
     const size_t zs = type_size(zt);
     const bool valued = !(s->sr.mult_op == GM_MULT_PAIR || s->sr.mult_op == GM_MULT_K);
-    bool want_v = false;
+    hipStream_t st = stream();
+    bool want_v = false, all_host = true;
     int64_t K = 0;
-    std::vector<BatchPart> pf, pv;
+    // the first-level requests' rows, stacked on the host in batch order and
+    // uploaded at once (F: one entry per source, iso one; V with its values)
+    std::vector<int64_t> hfp(1, 0), hfj, hvp(1, 0), hvj;
+    std::vector<uint8_t> hvx;
     for (auto &r : batch)
     {
         want_v |= r->needs_visited();
-        const SvHold &F = *r->F.h, &V = *r->V.h;
-        pf.push_back(BatchPart{F.M.p + r->F.row0, F.M.j, F.M.x, r->k, K, 0, 0, F.M.iso, 0});
-        pv.push_back(BatchPart{V.M.p + r->V.row0, V.M.j, V.M.x, r->k, K, 0, 0, V.M.iso, 0});
         K += r->k;
+        if (r->F.h) { all_host = false; continue; }
+        for (int64_t t = 0; t < r->k; t++)
+        {
+            hfj.push_back(r->src[t]);
+            hfp.push_back((int64_t) hfj.size());
+            hvp.push_back(hvp.back() + (r->hvp[t + 1] - r->hvp[t]));
+        }
+        hvj.insert(hvj.end(), r->hvj.begin(), r->hvj.end());
+        hvx.insert(hvx.end(), r->hvx.begin(), r->hvx.end());
+    }
+    const int64_t Kh = (int64_t) hfp.size() - 1;
+    DevBuf uFp, uFj, uFx, uVp, uVj, uVx;
+    if (Kh > 0)
+    {
+        const int64_t fz = hfp[Kh], vz = hvp[Kh];
+        if (!uFp.alloc(sizeof(int64_t) * (Kh + 1)) || !uFj.alloc(sizeof(int64_t) * (fz > 0 ? fz : 1)) ||
+            !uFx.alloc(zs > 0 ? zs : 8) || !uVp.alloc(sizeof(int64_t) * (Kh + 1)) ||
+            !uVj.alloc(sizeof(int64_t) * (vz > 0 ? vz : 1)) || !uVx.alloc(hvx.size() > 0 ? hvx.size() : 8))
+            return set_error(GM_OUT_OF_MEMORY, "server: upload of new requests");
+        std::vector<uint8_t> one(zs > 0 ? zs : 1);
+        sv_value(one.data(), zt, 1);
+        GM_HIP(hipMemcpyAsync(uFp.ptr, hfp.data(), sizeof(int64_t) * (Kh + 1), hipMemcpyHostToDevice, st), "F rows");
+        if (fz > 0) GM_HIP(hipMemcpyAsync(uFj.ptr, hfj.data(), sizeof(int64_t) * fz, hipMemcpyHostToDevice, st), "F");
+        GM_HIP(hipMemcpyAsync(uFx.ptr, one.data(), zs, hipMemcpyHostToDevice, st), "F value");
+        GM_HIP(hipMemcpyAsync(uVp.ptr, hvp.data(), sizeof(int64_t) * (Kh + 1), hipMemcpyHostToDevice, st), "V rows");
+        if (vz > 0) GM_HIP(hipMemcpyAsync(uVj.ptr, hvj.data(), sizeof(int64_t) * vz, hipMemcpyHostToDevice, st), "V");
+        if (!hvx.empty())
+            GM_HIP(hipMemcpyAsync(uVx.ptr, hvx.data(), hvx.size(), hipMemcpyHostToDevice, st), "V values");
+        // the pageable sources above must be read before they go out of scope
+        GM_HIP(hipStreamSynchronize(st), "server upload sync");
     }
     const int64_t n = s->graph.ncols;
     DevBuf tf, Pf, Jf, Xf, tv, Pv, Jv, Xv;
     gm_csr Fs{}, Vs{};
-    GM_TRY(bt_stack(pf, K, n, valued, zt, tf, Pf, Jf, Xf, &Fs));
-    GM_TRY(bt_stack(pv, K, n, want_v, zt, tv, Pv, Jv, Xv, &Vs));
+    if (all_host)
+    {
+        // every request starts here: the upload is the stacked operand
+        Fs.nrows = K; Fs.ncols = n; Fs.p = uFp.as<int64_t>(); Fs.j = uFj.as<int64_t>();
+        Fs.x = uFx.ptr; Fs.iso = 1; Fs.type = zt;
+        Vs.nrows = K; Vs.ncols = n; Vs.p = uVp.as<int64_t>(); Vs.j = uVj.as<int64_t>();
+        Vs.x = uVx.ptr; Vs.iso = 0; Vs.type = zt;
+        Xv.ptr = nullptr;
+    }
+    else
+    {
+        std::vector<BatchPart> pf, pv;
+        int64_t row = 0, hrow = 0;
+        for (auto &r : batch)
+        {
+            if (r->F.h)
+            {
+                const SvHold &F = *r->F.h, &V = *r->V.h;
+                pf.push_back(BatchPart{F.M.p + r->F.row0, F.M.j, F.M.x, r->k, row, 0, 0, F.M.iso, 0});
+                pv.push_back(BatchPart{V.M.p + r->V.row0, V.M.j, V.M.x, r->k, row, 0, 0, V.M.iso, 0});
+            }
+            else
+            {
+                pf.push_back(BatchPart{uFp.as<int64_t>() + hrow, uFj.as<int64_t>(), uFx.ptr, r->k, row, 0, 0, 1, 0});
+                pv.push_back(BatchPart{uVp.as<int64_t>() + hrow, uVj.as<int64_t>(), uVx.ptr, r->k, row, 0, 0, 0, 0});
+                hrow += r->k;
+            }
+            row += r->k;
+        }
+        GM_TRY(bt_stack(pf, K, n, valued, zt, tf, Pf, Jf, Xf, &Fs));
+        GM_TRY(bt_stack(pv, K, n, want_v, zt, tv, Pv, Jv, Xv, &Vs));
+    }
     if (!valued) Fs.x = nullptr;
     auto C = std::make_shared<SvHold>();
     GM_TRY(gm_saxpy3(&C->M, &Vs, true, true, &Fs, &s->graph, &s->sr, nullptr));
@@ -178,8 +259,8 @@ static int sv_level(gm_server *s, std::vector<std::shared_ptr<SvRequest>> &batch
         // V + C: the product is outside V (the mask), so this is a merge of
         // disjoint rows; FIRST keeps V's parents on an intersection
         Vn = std::make_shared<SvHold>();
-        gm_csr vc = Vs, tc;
-        vc.x = Xv.ptr;
+        gm_csr vc = Vs, tc{};
+        if (!all_host) vc.x = Xv.ptr;
         vc.iso = 0;
         tc.nrows = C->M.nrows; tc.ncols = C->M.ncols; tc.p = C->M.p; tc.j = C->M.j;
         tc.x = C->M.x; tc.iso = C->M.iso; tc.type = zt;
@@ -190,7 +271,6 @@ static int sv_level(gm_server *s, std::vector<std::shared_ptr<SvRequest>> &batch
     DevBuf rf;
     if (!rf.alloc(sizeof(int64_t) * (K > 0 ? K : 1)))
         return set_error(GM_OUT_OF_MEMORY, "server: flop bound workspace");
-    hipStream_t st = stream();
     if (K > 0)
     {
         int64_t g = (K * 64 + 255) / 256;
@@ -214,6 +294,9 @@ static int sv_level(gm_server *s, std::vector<std::shared_ptr<SvRequest>> &batch
         // visited set grows by it
         r->F = r->C;
         r->flops = f;
+        r->hvp = std::vector<int64_t>();
+        r->hvj = std::vector<int64_t>();
+        r->hvx = std::vector<uint8_t>();
         if (Vn)
         {
             // rows of V + C: V's entries plus C's (disjoint)
@@ -351,41 +434,6 @@ static int sv_check_visited(const int64_t *vp, const int64_t *vj, int64_t k, int
     return GM_SUCCESS;
 }
 
-// a host CSR uploaded into a new holder (values of the semiring's type)
-static int sv_upload(std::shared_ptr<SvHold> &h, int64_t k, int64_t ncols, const std::vector<int64_t> &p,
-                     const std::vector<int64_t> &j, const std::vector<uint8_t> &x, bool iso, int32_t zt)
-{
-    h = std::make_shared<SvHold>();
-    const int64_t nz = p[k];
-    h->M.nrows = k;
-    h->M.ncols = ncols;
-    h->M.nvals = nz;
-    h->M.iso = iso ? 1 : 0;
-    h->M.type = zt;
-    h->M.p = (int64_t *) gm_malloc(sizeof(int64_t) * (k + 1));
-    h->M.j = (int64_t *) gm_malloc(sizeof(int64_t) * (nz > 0 ? nz : 1));
-    h->M.x = gm_malloc(x.size() > 0 ? x.size() : 8);
-    if (!h->M.p || !h->M.j || !h->M.x) return set_error(GM_OUT_OF_MEMORY, "gm_server_submit: upload");
-    GM_TRY(gm_memcpy_h2d(h->M.p, p.data(), sizeof(int64_t) * (k + 1)));
-    if (nz > 0) GM_TRY(gm_memcpy_h2d(h->M.j, j.data(), sizeof(int64_t) * nz));
-    if (!x.empty()) GM_TRY(gm_memcpy_h2d(h->M.x, x.data(), x.size()));
-    return GM_SUCCESS;
-}
-
-// value v of type zt as bytes
-static void sv_value(uint8_t *out, int32_t zt, int64_t v)
-{
-    switch (zt)
-    {
-        case GM_FP64: { double d = (double) v; memcpy(out, &d, 8); break; }
-        case GM_FP32: { float f = (float) v; memcpy(out, &f, 4); break; }
-        case GM_INT64: case GM_UINT64: memcpy(out, &v, 8); break;
-        case GM_INT32: case GM_UINT32: { int32_t w = (int32_t) v; memcpy(out, &w, 4); break; }
-        case GM_INT16: case GM_UINT16: { int16_t w = (int16_t) v; memcpy(out, &w, 2); break; }
-        default: out[0] = (uint8_t) (v != 0);
-    }
-}
-
 extern "C" {
 
 gm_server *gm_server_create(const gm_csr *graph, const gm_semiring *sr, const gm_server_config *cfg)
@@ -450,14 +498,12 @@ int64_t gm_server_submit_bfs(gm_server *s, const int64_t *sources, int64_t k, co
         r->max_steps = opts->max_steps;
         r->keep_visited = opts->keep_visited != 0;
     }
-    // frontier F (iso one) and visited set V in HBM
+    // frontier F (the sources) and visited set V stay on the host until the
+    // batch that runs the request's first level uploads them with the others
     const int32_t zt = s->sr.ztype;
     const size_t zs = type_size(zt);
     std::vector<int64_t> fp(k + 1), fj(r->src);
     for (int64_t t = 0; t <= k; t++) fp[t] = t;
-    std::vector<uint8_t> one(zs);
-    sv_value(one.data(), zt, 1);
-    GM_TRY(sv_upload(r->F.h, k, n, fp, fj, one, true, zt));
     r->F.nnz = k;
     // V's values: the parent, i.e. the source for its own row (positional
     // semirings) or one; -1 for the entries the caller marked visited
@@ -481,8 +527,10 @@ int64_t gm_server_submit_bfs(gm_server *s, const int64_t *sources, int64_t k, co
         vx.resize(zs * k);
         for (int64_t t = 0; t < k; t++) sv_value(vx.data() + t * zs, zt, positional ? r->src[t] : 1);
     }
-    GM_TRY(sv_upload(r->V.h, k, n, vp, vj, vx, false, zt));
     r->V.nnz = vp[k];
+    r->hvp = std::move(vp);
+    r->hvj = std::move(vj);
+    r->hvx = std::move(vx);
     r->t_arrive = now_us();
     int64_t id;
     {

@@ -316,6 +316,15 @@ int gm_dot2(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_struct, co
 // GB_nvec_nonempty, Source/hyper/GB_nvec_nonempty.c:17), for the dot2 rule
 int gm_nonempty_rows(int64_t *count, const gm_csr *A);
 
+// The list lengths a dot3 call C<M> = A*B' reads under the two models of its
+// algorithmic bytes: *min_sum = sum over M's entries (i,j) of
+// min(|A(i,:)|, |B(j,:)|) (the shorter list probed into the longer one's
+// index) and *len_sum = sum of |A(i,:)| + |B(j,:)| (both lists merged, as
+// the reference's GB_AxB_dot_cij.c:625-649 reads them).  Measurement only:
+// no counterpart in the reference.
+int gm_dot3_work(int64_t *min_sum, int64_t *len_sum, const gm_csr *M, const gm_csr *A,
+                 const gm_csr *B);
+
 //------------------------------------------------------------------------------
 // saxpy3: C = A*B, C<M> = A*B, C<!M> = A*B (row-wise Gustavson / hash)
 //------------------------------------------------------------------------------

@@ -10,7 +10,7 @@ import pytest
 import pyoracle as O
 from graphblas_amd import capi
 from graphblas_amd import workloads as W
-from helpers import SR, assert_csr_equal, gm_dot3, power_law_csr, rand_csr
+from helpers import SR, DevCsr, assert_csr_equal, gm_dot3, power_law_csr, rand_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -259,3 +259,22 @@ def test_dot3_row_mode_wide_rows(grb):
     ref = O.dot3("min_plus_int32", M, A, B)
     got = gm_dot3("GrB_MIN_PLUS_SEMIRING_INT32", M, A, B, params=(0, 0, 0, 0, 0, 1))
     assert_csr_equal(got, ref, what="wide rows")
+
+
+def test_dot3_work_model(grb):
+    """gm_dot3_work (the list lengths behind the roofline's algorithmic bytes)
+    against numpy on a power-law mask."""
+    import ctypes
+    rng = np.random.default_rng(21)
+    A = power_law_csr(600, 12, rng)
+    B = power_law_csr(500, 12, rng)
+    M = rand_csr(600, 500, 0.05, rng)
+    la, lb = np.diff(A.p), np.diff(B.p)
+    rows = np.repeat(np.arange(M.nrows), np.diff(M.p))
+    a, b = la[rows], lb[M.j]
+    dM, dA, dB = DevCsr(M, 1), DevCsr(A, 1), DevCsr(B, 1)
+    mn, ln = ctypes.c_int64(), ctypes.c_int64()
+    rc = capi.lib().gm_dot3_work(ctypes.byref(mn), ctypes.byref(ln), ctypes.byref(dM.view),
+                                 ctypes.byref(dA.view), ctypes.byref(dB.view))
+    assert rc == 0, capi.last_error()
+    assert mn.value == int(np.minimum(a, b).sum()) and ln.value == int((a + b).sum())
