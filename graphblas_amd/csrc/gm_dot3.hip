@@ -28,6 +28,7 @@
 //          rows that start in the tile.  The result has no zombies.
 
 #include <cstring>
+#include <type_traits>
 #include <rocprim/device/device_radix_sort.hpp>
 #include "gm_internal.h"
 #include "gm_block.h"
@@ -138,7 +139,8 @@ __device__ __forceinline__ unsigned long long wave_append(bool pred, unsigned lo
 struct Pair
 {
     int64_t pa, pa_end, pb, pb_end;
-    int64_t off, lo, hi;          // probe index of the longer row
+    int64_t off, lo, hi;          // probe index of the longer row (merges: lo and hi are
+                                  // the last columns of A's and B's rows)
     int64_t slo, shi;             // first / last column of the probing (short) row
     int probe;
     int hlg;                      // 0: bitmap, else log2 of the hash size
@@ -184,6 +186,7 @@ __device__ __forceinline__ int64_t d3_pair(const Dot3Args &g, int64_t i, int64_t
             P.slo = db.lo; P.shi = db.hi;
             return b;
         }
+        P.lo = da.hi; P.hi = db.hi;    // merges: the rows' last columns
         return d3_work_merge(a, b);
     }
     P.pa = g.Ap[i]; P.pa_end = g.Ap[i + 1];
@@ -218,6 +221,7 @@ __device__ __forceinline__ int64_t d3_pair(const Dot3Args &g, int64_t i, int64_t
         P.slo = blo; P.shi = bhi;
         return b;
     }
+    P.lo = ahi; P.hi = bhi;
     return d3_work_merge(a, b);
 }
 
@@ -338,8 +342,8 @@ __device__ __forceinline__ void d3_trim_short(const Dot3Args &g, const Pair &P, 
 // evenly spaced positions and a ballot narrows the range G-fold, so the
 // search costs log_G(hi-lo) rounds of independent loads instead of
 // log_2(hi-lo) dependent ones.  Every lane of the team must call it.
-template <int G>
-__device__ __forceinline__ int64_t team_lower_bound(const int64_t *X, int64_t lo, int64_t hi, int64_t v,
+template <int G, class T = int64_t>
+__device__ __forceinline__ int64_t team_lower_bound(const T *X, int64_t lo, int64_t hi, int64_t v,
                                                     int lane)
 {
     static_assert(G <= 64, "one wavefront");
@@ -350,7 +354,7 @@ __device__ __forceinline__ int64_t team_lower_bound(const int64_t *X, int64_t lo
         const int64_t len = hi - lo;
         const int64_t stride = (len + G - 1) / G;
         const int64_t pos = lo + (int64_t) lane * stride;
-        const bool less = pos < hi && X[pos] < v;
+        const bool less = pos < hi && (int64_t) X[pos] < v;
         const int cnt = __popcll((__ballot(less) >> base) & tmask);
         if (stride == 1) return lo + cnt;
         if (cnt == 0) return lo;
@@ -403,6 +407,55 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
     const int64_t pa_end = P.pa_end, pb_end = P.pb_end;
     const int64_t a = pa_end - pa, b = pb_end - pb;
     const bool trimA = a > 8 * b, trimB = b > 8 * a;
+    if (!trimA && !trimB)
+    {
+        // blocks of four from each row, their loads issued together; all
+        // pairs of a block compared in registers, then the block with the
+        // smaller last column (or both) moves on.  Matches come in
+        // ascending column order, as in the element-wise merge.
+        const int64_t ahi = P.lo, bhi = P.hi;
+        while (pa < pa_end && pb < pb_end)
+        {
+            int64_t xa[4], xb[4];
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                xa[u] = Aj[pa + u < pa_end ? pa + u : pa_end - 1];
+                xb[u] = Bj[pb + u < pb_end ? pb + u : pb_end - 1];
+            }
+            // past a row's end: sentinels that match nothing
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                if (pa + u >= pa_end) xa[u] = INT64_MAX;
+                if (pb + u >= pb_end) xb[u] = INT64_MAX - 1;
+            }
+            if (xa[0] > bhi || xb[0] > ahi) break;
+            bool stop = false;
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+            {
+                #pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    if (xa[u] == xb[v] && !stop)
+                    {
+                        Z t = S::mul(S::geta(Ax, pa + u, g.a_iso), S::geta(Bx, pb + v, g.b_iso), xa[u]);
+                        if (!e) { z = t; e = true; } else z = S::upd(z, t);
+                        if constexpr (S::is_any) stop = true;
+                        if (S::terminal(z)) stop = true;
+                    }
+                }
+            }
+            if (stop) break;
+            const int64_t am = xa[3], bm = xb[3];
+            if (am <= bm) pa += 4;
+            if (bm <= am) pb += 4;
+        }
+        if constexpr (S::plus_pair_wide) e = (z != 0);
+        out = z;
+        return e;
+    }
     int64_t ia = Aj[pa], ib = Bj[pb];
     while (true)
     {
@@ -895,32 +948,41 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint
 // of the slice take the global path of the wavefront kernel.  Slices are
 // handed out per XCD region, as in the grouped team kernels, so a long
 // row's bitmap chunks are read from one L2.
-constexpr int LC_BITS = 1 << 20;              // columns per chunk: 128 KB of LDS
+constexpr int64_t LC_POS_MAX = (int64_t) 1 << 28;   // chunk offsets kept (1 GB)
+constexpr int LC_WAVE_G = 16;                 // lanes per entry of the wavefront bin
+constexpr bool LC_NARROW = true;              // 32-bit short-row columns when they fit
+constexpr int LC_LG = 20;
+constexpr int LC_BITS = 1 << LC_LG;           // columns per chunk: 128 KB of LDS
 constexpr int LC_WORDS = LC_BITS / 64;
 constexpr int LC_SLICE = 512;
 constexpr int LC_GMIN = 8;
-constexpr int LC_UNROLL = 8;
+constexpr int LC_UNROLL = 16;
 constexpr int LC_NT = 1024;                   // 16 wavefronts share a chunk (one workgroup per CU)
 
-template <class S, int G>
+template <class S, int G, bool N32>
 __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *list,
                                                     const uint64_t *keys, int64_t count,
-                                                    unsigned long long *ctr)
+                                                    unsigned long long *ctr, const uint32_t *posA,
+                                                    const uint32_t *posB, int ncp, const uint32_t *A32,
+                                                    const uint32_t *B32)
 {
     using Z = typename S::Z;
     static_assert(S::pattern, "LDS probing covers the pattern semirings");
+    static_assert(G <= 64 && (64 % G) == 0, "teams inside one wavefront");
     constexpr int NW = LC_NT / G;            // teams of G lanes, one entry each
-    __shared__ uint64_t s_bits[LC_WORDS];
-    __shared__ uint64_t s_key[LC_SLICE];
-    __shared__ int64_t s_cur[LC_SLICE], s_end[LC_SLICE], s_nx[LC_SLICE];
-    __shared__ Z s_z[LC_SLICE];
-    __shared__ uint32_t s_p[LC_SLICE];
-    __shared__ uint8_t s_st[LC_SLICE];        // bit 0: result exists, 1: LDS entry, 2: visited
-    __shared__ int64_t s_slice, s_lcur, s_lnext;
-    __shared__ int s_ge;
-    const int lane = threadIdx.x & (G - 1), wv = threadIdx.x / G;
+    constexpr int UN = N32 ? LC_UNROLL : LC_UNROLL / 2;   // windows per step (64-bit: registers)
     const int tbase = (int) (threadIdx.x & 63) & ~(G - 1);
     const unsigned long long tmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    __shared__ uint64_t s_bits[LC_WORDS];
+    __shared__ uint64_t s_key[LC_SLICE];
+    __shared__ int64_t s_srow[LC_SLICE], s_rp0[LC_SLICE];   // the short row and its start
+    __shared__ uint32_t s_qs[LC_SLICE], s_qe[LC_SLICE];       // its part in the current chunk
+    __shared__ Z s_z[LC_SLICE];
+    __shared__ uint32_t s_p[LC_SLICE];
+    __shared__ uint8_t s_st[LC_SLICE];        // bit 0: result exists, 1: LDS entry
+    __shared__ int64_t s_slice;
+    __shared__ int s_ge;
+    const int lane = threadIdx.x & (G - 1), wv = threadIdx.x / G;
     const int64_t nslices = (count + LC_SLICE - 1) / LC_SLICE;
     const int x0 = (int) (blockIdx.x & 7);
     for (int k = 0; k < 8; k++)
@@ -965,9 +1027,8 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                         if (!P.empty && P.probe == (rowA ? 1 : 2))
                         {
                             st = 2;
-                            s_cur[t] = rowA ? P.pb : P.pa;
-                            s_end[t] = rowA ? P.pb_end : P.pa_end;
-                            s_nx[t] = P.slo;
+                            s_srow[t] = rowA ? g.Mj[p] : i;
+                            s_rp0[t] = rowA ? P.pb : P.pa;
                             s_z[t] = S::identity();
                         }
                     }
@@ -991,69 +1052,118 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                 }
                 if (use_lds)
                 {
+                    // the long row's bitmap one aligned chunk of LC_BITS
+                    // columns at a time; every row's position at each chunk
+                    // boundary was found beforehand (pos arrays), so an
+                    // entry's visit reads exactly its short row's elements
+                    // inside the chunk and the long row's part is known
                     const int64_t *Sj = rowA ? g.Bj : g.Aj;
+                    const uint32_t *Sj32 = rowA ? B32 : A32;   // N32: the same columns, 32-bit
                     const int64_t *Lj = rowA ? g.Aj : g.Bj;
-                    const int64_t lend = d.p0 + d.len;
-                    const int64_t nwords = (((int64_t) d.hi - (int64_t) d.lo) >> 6) + 1;
-                    if (threadIdx.x == 0) s_lcur = d.p0;
-                    for (int64_t c0 = 0; c0 < nwords; c0 += LC_WORDS)
+                    const uint32_t *posS = rowA ? posB : posA;
+                    const uint32_t *posL = rowA ? posA : posB;
+                    const int64_t c_lo = (int64_t) d.lo >> LC_LG, c_hi = (int64_t) d.hi >> LC_LG;
+                    for (int64_t c = c_lo; c <= c_hi; c++)
                     {
-                        const int64_t cw = (nwords - c0) < LC_WORDS ? (nwords - c0) : LC_WORDS;
-                        const int64_t cbeg = (int64_t) d.lo + c0 * 64, cend = cbeg + cw * 64;
-                        __syncthreads();
-                        for (int64_t t = threadIdx.x; t < cw; t += LC_NT) s_bits[t] = 0;
-                        if (threadIdx.x == 0) s_lnext = lend;
-                        __syncthreads();
-                        // the long row's columns in [cbeg, cend): a contiguous run
-                        // from s_lcur on; each thread stops at its first column
-                        // past the chunk, the smallest such index starts the next
-                        const int64_t lc = s_lcur;
-                        for (int64_t e = lc + threadIdx.x; e < lend; e += LC_NT)
-                        {
-                            const int64_t x = Lj[e];
-                            if (x >= cend)
+                        const int64_t cbeg = c << LC_LG;
+                        const int64_t clo = (int64_t) d.lo > cbeg ? (int64_t) d.lo : cbeg;
+                        const int64_t chi = (int64_t) d.hi < cbeg + LC_BITS - 1 ? (int64_t) d.hi
+                                                                             : cbeg + LC_BITS - 1;
+                        const int wlo = (int) ((clo - cbeg) >> 6), whi = (int) ((chi - cbeg) >> 6) + 1;
+                        __syncthreads();              // the previous chunk's probes are done
+                        for (int w = wlo + (int) threadIdx.x; w < whi; w += LC_NT) s_bits[w] = 0;
+                        // every entry's short-row part in this chunk, loaded
+                        // together while the bitmap is built
+                        for (int t = gs + (int) threadIdx.x; t < ge; t += LC_NT)
+                            if (s_st[t] & 2)
                             {
-                                atomicMin((unsigned long long *) &s_lnext, (unsigned long long) e);
-                                break;
+                                const uint32_t *ps = posS + s_srow[t] * ncp + c;
+                                s_qs[t] = ps[0];
+                                s_qe[t] = ps[1];
                             }
-                            const int64_t dd = x - cbeg;
+                        __syncthreads();
+                        const int64_t l0 = d.p0 + posL[rr * ncp + c], l1 = d.p0 + posL[rr * ncp + c + 1];
+                        for (int64_t e = l0 + threadIdx.x; e < l1; e += LC_NT)
+                        {
+                            const int64_t dd = Lj[e] - cbeg;
                             atomicOr((unsigned long long *) &s_bits[dd >> 6], 1ull << (dd & 63));
                         }
                         __syncthreads();
-                        if (threadIdx.x == 0) s_lcur = s_lnext;
                         for (int t = gs + wv; t < ge; t += NW)
                         {
                             const uint8_t st = s_st[t];
-                            if (!(st & 2) || s_nx[t] >= cend) continue;
-                            int64_t cur = s_cur[t];
-                            const int64_t end = s_end[t];
-                            // first visit: skip the short row's elements below the long row
-                            if (!(st & 4) && s_nx[t] < (int64_t) d.lo)
-                                cur = team_lower_bound<G>(Sj, cur, end, (int64_t) d.lo, lane);
+                            if (!(st & 2)) continue;
+                            const uint32_t rs = s_qs[t], re = s_qe[t];
+                            if (rs >= re) continue;
+                            const int64_t rp0 = s_rp0[t];
+                            int64_t qs = rp0 + rs;
+                            const int64_t qe = rp0 + re;
+                            // the long row's first chunk: skip the short row's
+                            // elements below the long row's first column
+                            if (c == c_lo && clo > cbeg)
+                            {
+                                if constexpr (N32) qs = team_lower_bound<G>(Sj32, qs, qe, clo, lane);
+                                else qs = team_lower_bound<G>(Sj, qs, qe, clo, lane);
+                            }
                             bool e = false;
                             Z z = S::identity();
-                            int64_t nx = INT64_MAX;
-                            // LC_UNROLL windows of G elements per step, their
-                            // loads issued together (elements past the chunk are
-                            // read again on the next visit, mostly from cache)
-                            while (cur < end)
+                            uint32_t cnt = 0;         // plus_pair: matches of this lane
+                            // UN windows of G elements per step, their
+                            // loads issued together; elements outside the long
+                            // row's range inside the chunk are skipped
+                            for (int64_t q = qs; q < qe; q += UN * G)
                             {
-                                int64_t xs[LC_UNROLL];
+                                // lanes past the end read the last element
+                                // again (same line): the loads carry no branch
+                                // and all issue before the first is used
+                                using CT = std::conditional_t<N32, uint32_t, int64_t>;
+                                CT raw[UN];
                                 #pragma unroll
-                                for (int u = 0; u < LC_UNROLL; u++)
+                                for (int u = 0; u < UN; u++)
                                 {
-                                    const int64_t q = cur + u * G + lane;
-                                    xs[u] = q < end ? Sj[q] : INT64_MAX;
+                                    const int64_t qq = q + u * G + lane;
+                                    const int64_t qc = qq < qe ? qq : qe - 1;
+                                    if constexpr (N32) raw[u] = Sj32[qc];
+                                    else raw[u] = Sj[qc];
                                 }
-                                int taken = 0;
-                                bool stop = false;
-                                #pragma unroll
-                                for (int u = 0; u < LC_UNROLL; u++)
+                                if constexpr (N32)
                                 {
-                                    const int64_t x = xs[u];
-                                    const bool in = x < cend;
-                                    if (in)
+                                    // 32-bit columns: one unsigned compare for the
+                                    // range, 32-bit words of the bitmap
+                                    const uint32_t c0 = (uint32_t) clo, span = (uint32_t) (chi - clo);
+                                    const uint32_t boff = (uint32_t) (clo - cbeg);
+                                    const uint32_t *bits32 = (const uint32_t *) s_bits;
+                                    #pragma unroll
+                                    for (int u = 0; u < UN; u++)
                                     {
+                                        const uint32_t d = raw[u] - c0;
+                                        if (q + u * G + lane >= qe || d > span) continue;
+                                        const uint32_t bi = d + boff;
+                                        const uint32_t hit = (bits32[bi >> 5] >> (bi & 31)) & 1u;
+                                        if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR) cnt += hit;
+                                        else if (hit)
+                                        {
+                                            const Z tz = S::mul(Z(1), Z(1), (int64_t) raw[u]);
+                                            if (!e) { z = tz; e = true; }
+                                            else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                                            else z = S::upd(z, tz);
+                                        }
+                                    }
+                                    const bool past = q + (UN - 1) * G + lane < qe &&
+                                                      raw[UN - 1] > (uint32_t) chi;
+                                    if ((__ballot(past) >> tbase) & tmask) break;
+                                }
+                                else
+                                {
+                                    int64_t xs[UN];
+                                    #pragma unroll
+                                    for (int u = 0; u < UN; u++)
+                                        xs[u] = (q + u * G + lane < qe) ? (int64_t) raw[u] : -1;
+                                    #pragma unroll
+                                    for (int u = 0; u < UN; u++)
+                                    {
+                                        const int64_t x = xs[u];
+                                        if (x < clo || x > chi) continue;
                                         const int64_t dd = x - cbeg;
                                         if ((s_bits[dd >> 6] >> (dd & 63)) & 1ull)
                                         {
@@ -1063,20 +1173,14 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                                             else z = S::upd(z, tz);
                                         }
                                     }
-                                    const int nin = __popcll((__ballot(in) >> tbase) & tmask);
-                                    if (!stop)
-                                    {
-                                        taken += nin;
-                                        if (nin < G)
-                                        {
-                                            nx = __shfl(x, nin, G);
-                                            stop = true;
-                                        }
-                                    }
+                                    // sorted: once the step's last window passes
+                                    // the long row's last column, nothing after
+                                    // it matches
+                                    if ((__ballot(xs[UN - 1] > chi) >> tbase) & tmask) break;
                                 }
-                                cur += taken;
-                                if (stop) break;
                             }
+                            if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+                                if (cnt) { e = true; z = (Z) cnt; }
                             team_reduce<S, G>(e, z);
                             if (lane == 0)
                             {
@@ -1084,9 +1188,7 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
                                 Z zz = s_z[t];
                                 combine<S>(ee, zz, e, z);
                                 s_z[t] = zz;
-                                s_st[t] = (uint8_t) ((st | 4) & ~1) | (ee ? 1 : 0);
-                                s_cur[t] = cur;
-                                s_nx[t] = cur < end ? nx : INT64_MAX;
+                                s_st[t] = (uint8_t) ((st & ~1) | (ee ? 1 : 0));
                             }
                         }
                     }
@@ -1817,6 +1919,43 @@ __global__ __launch_bounds__(D3_NT) void k_d3_bm_build(const int64_t *__restrict
     }
 }
 
+// 32-bit copy of a column index array (columns below 2^32)
+__global__ __launch_bounds__(D3_NT) void k_d3_narrow(const int64_t *__restrict__ X, int64_t n,
+                                                     uint32_t *__restrict__ out)
+{
+    for (int64_t t = (int64_t) blockIdx.x * D3_NT + threadIdx.x; t < n; t += (int64_t) gridDim.x * D3_NT)
+        out[t] = (uint32_t) X[t];
+}
+
+// Each row's offsets at the LDS chunk boundaries: pos[r*ncp + c] is the
+// number of the row's entries whose column is below c*LC_BITS (so entry 0
+// is 0 and entry ncp-1 is the row length).  One thread per (row, boundary),
+// a binary search inside the row.
+__global__ __launch_bounds__(D3_NT) void k_d3_chunkpos(const int64_t *__restrict__ Xp,
+                                                       const int64_t *__restrict__ Xj, int64_t nrows,
+                                                       int ncp, uint32_t *__restrict__ pos)
+{
+    const int64_t total = nrows * ncp;
+    for (int64_t t = (int64_t) blockIdx.x * D3_NT + threadIdx.x; t < total;
+         t += (int64_t) gridDim.x * D3_NT)
+    {
+        const int64_t r = t / ncp;
+        const int64_t c = t - r * ncp;
+        const int64_t p0 = Xp[r], p1 = Xp[r + 1];
+        const int64_t key = c << LC_LG;
+        int64_t lo = p0, hi = p1;
+        if (c == 0) hi = p0;
+        else if (c == ncp - 1) lo = p1;
+        while (lo < hi)
+        {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (Xj[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        pos[t] = (uint32_t) (lo - p0);
+    }
+}
+
 // one summary per row (off: the probe index offsets, or NULL when the
 // operand has none; rows of length >= T have an index)
 __global__ __launch_bounds__(D3_NT) void k_d3_desc(const int64_t *__restrict__ Xp,
@@ -1956,13 +2095,17 @@ static int dot3_bitmaps(Dot3Args &g, const gm_csr *A, const gm_csr *B, bool need
 // of (key, entry) pairs) and run them with the XCD-regioned kernels.  The
 // bins' keys must have been written by phase1 (g.bkeys != NULL).
 template <class S>
-static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
+static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
 {
+    const int64_t nkeys = A->nrows + B->nrows;
     hipStream_t st = stream();
-    unsigned long long *hc = (unsigned long long *) pinned_scratch(4 * sizeof(unsigned long long));
+    unsigned long long *hc = (unsigned long long *) pinned_scratch(6 * sizeof(unsigned long long));
     if (!hc) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
     GM_HIP(hipMemcpyAsync(hc, g.bin_cnt, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st),
            "bin counts");
+    // the operands' entry counts (32-bit column copies for the chunk kernel)
+    GM_HIP(hipMemcpyAsync(hc + 4, A->p + A->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(A)");
+    GM_HIP(hipMemcpyAsync(hc + 5, B->p + B->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(B)");
     GM_HIP(hipStreamSynchronize(st), "bin counts sync");
     const int64_t cnt[3] = {(int64_t) hc[0], (int64_t) hc[1], (int64_t) hc[2]};
     // every row spans at most L1_BITS columns: whole bitmaps fit in LDS
@@ -1990,6 +2133,50 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
     // the wavefront and workgroup bins probe bitmaps from LDS when the
     // semiring needs no positions and the row summaries exist
     const bool lds = S::pattern && g.dA != nullptr && g.bA.words != nullptr;
+    // the chunk kernel reads every row's offsets at the chunk boundaries;
+    // without room for them (very wide matrices) those bins take the
+    // global-memory kernels
+    DevBuf posA, posB;
+    const int ncp = (int) ((A->ncols + LC_BITS - 1) >> LC_LG) + 1;
+    const bool shared = (A->p == B->p && A->j == B->j && A->nrows == B->nrows);
+    bool lds_chunk = false;
+    if (lds && (cnt[2] > 0 || (cnt[1] > 0 && !small)))
+    {
+        const int64_t na = A->nrows * ncp, nb = shared ? 0 : B->nrows * ncp;
+        if (na + nb <= LC_POS_MAX && posA.alloc(sizeof(uint32_t) * (na > 0 ? na : 1)) &&
+            (shared || posB.alloc(sizeof(uint32_t) * (nb > 0 ? nb : 1))))
+        {
+            ProfScope ps("dot3_chunkpos");
+            const unsigned cap = (unsigned) cu_count() * 32;
+            k_d3_chunkpos<<<grid_for(na, D3_NT, cap), D3_NT, 0, st>>>(A->p, A->j, A->nrows, ncp,
+                                                                      posA.as<uint32_t>());
+            if (!shared)
+                k_d3_chunkpos<<<grid_for(nb, D3_NT, cap), D3_NT, 0, st>>>(B->p, B->j, B->nrows, ncp,
+                                                                          posB.as<uint32_t>());
+            GM_HIP(hipGetLastError(), "k_d3_chunkpos");
+            lds_chunk = true;
+        }
+    }
+    const uint32_t *pA = posA.as<uint32_t>(), *pB = shared ? pA : posB.as<uint32_t>();
+    // the short rows' columns as 32-bit words halve the chunk kernel's reads
+    DevBuf narA, narB;
+    bool n32 = false;
+    if (lds_chunk && LC_NARROW && g.cols32)
+    {
+        const int64_t nza = (int64_t) hc[4], nzb = shared ? 0 : (int64_t) hc[5];
+        if (narA.alloc(sizeof(uint32_t) * (nza > 0 ? nza : 1)) &&
+            (shared || narB.alloc(sizeof(uint32_t) * (nzb > 0 ? nzb : 1))))
+        {
+            ProfScope ps("dot3_narrow");
+            const unsigned cap = (unsigned) cu_count() * 32;
+            k_d3_narrow<<<grid_for(nza, D3_NT, cap), D3_NT, 0, st>>>(A->j, nza, narA.as<uint32_t>());
+            if (!shared)
+                k_d3_narrow<<<grid_for(nzb, D3_NT, cap), D3_NT, 0, st>>>(B->j, nzb, narB.as<uint32_t>());
+            GM_HIP(hipGetLastError(), "k_d3_narrow");
+            n32 = true;
+        }
+    }
+    const uint32_t *nA = narA.as<uint32_t>(), *nB = shared ? nA : narB.as<uint32_t>();
     for (int b = 0; b < 3; b++)
     {
         if (cnt[b] == 0) continue;
@@ -2033,14 +2220,25 @@ static int dot3_grouped_teams(Dot3Args &g, int64_t nkeys)
                 g, tv.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
             GM_HIP(hipGetLastError(), "k_dot3_team_grp16");
         }
-        else if (lds)
+        else if (lds_chunk)
         {
             if constexpr (S::pattern)
             {
                 ProfScope ps(b == 1 ? "dot3_wave_lds" : "dot3_block_lds");
-                k_dot3_lds<S, 64><<<(unsigned) (cu_count() * 1), LC_NT, 0, st>>>(
-                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b],
-                    ctr.as<unsigned long long>() + (b == 1 ? 24 : 32));
+                const unsigned grid = (unsigned) cu_count();
+                unsigned long long *c = ctr.as<unsigned long long>() + (b == 1 ? 24 : 32);
+                if (b == 1 && n32)
+                    k_dot3_lds<S, LC_WAVE_G, true><<<grid, LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b], c, pA, pB, ncp, nA, nB);
+                else if (b == 1)
+                    k_dot3_lds<S, LC_WAVE_G, false><<<grid, LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b], c, pA, pB, ncp, nA, nB);
+                else if (n32)
+                    k_dot3_lds<S, 64, true><<<grid, LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b], c, pA, pB, ncp, nA, nB);
+                else
+                    k_dot3_lds<S, 64, false><<<grid, LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[b], c, pA, pB, ncp, nA, nB);
                 GM_HIP(hipGetLastError(), "k_dot3_lds");
             }
         }
@@ -2178,7 +2376,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
         }
         if (grouped)
         {
-            GM_TRY(dot3_grouped_teams<S>(g, A->nrows + B->nrows));
+            GM_TRY(dot3_grouped_teams<S>(g, A, B));
         }
         else
         {
