@@ -251,6 +251,44 @@ __device__ __forceinline__ int64_t bm_find(const RowBits &bm, const Pair &P, int
     else return p0 + (int64_t) bm.rank[P.off + (int64_t) (d >> 6)] + __popcll(w & (bit - 1));
 }
 
+// Two halves of bm_find, so that several probes' first reads can be issued
+// before any is used: the index unit x lands on (a bitmap word or its home
+// hash slot; unit 0 of the row's index when x is outside the row's range,
+// a safe read whose value is ignored), then the position from that unit
+// (hash: further slots on a collision).
+__device__ __forceinline__ uint64_t bm_first(const RowBits &bm, const Pair &P, int64_t x)
+{
+    const bool inr = x >= P.lo && x <= P.hi;
+    int64_t u = 0;
+    if (inr) u = P.hlg ? (int64_t) hx_hash((uint32_t) x, P.hlg) : (int64_t) ((uint64_t) (x - P.lo) >> 6);
+    return bm.words[P.off + u];
+}
+
+template <bool NEED_POS>
+__device__ __forceinline__ int64_t bm_resolve(const RowBits &bm, const Pair &P, int64_t p0, int64_t x,
+                                              uint64_t w)
+{
+    if (x < P.lo || x > P.hi) return -1;
+    if (P.hlg)
+    {
+        const uint64_t *T = bm.words + P.off;
+        const uint32_t mask = (1u << P.hlg) - 1u;
+        uint32_t h = hx_hash((uint32_t) x, P.hlg);
+        while (true)
+        {
+            if (w == HX_EMPTY) return -1;
+            if ((int64_t) (w >> 32) == x) return p0 + (int64_t) (w & 0xffffffffull);
+            h = (h + 1) & mask;
+            w = T[h];
+        }
+    }
+    const uint64_t d = (uint64_t) (x - P.lo);
+    const uint64_t bit = 1ull << (d & 63);
+    if (!(w & bit)) return -1;
+    if constexpr (!NEED_POS) return p0;
+    else return p0 + (int64_t) bm.rank[P.off + (int64_t) (d >> 6)] + __popcll(w & (bit - 1));
+}
+
 // probe elements q0, q0+step, ... < q1 of the short row in the long row's
 // index; the iterations are independent, so several probes are in flight
 // STOP: the lane leaves after the first step that passes the long row's
@@ -275,20 +313,29 @@ __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P,
         // together; the lane stops after a step that passed the range
         for (int64_t q = q0; q < q1; q += 4 * step)
         {
+            // past the end: the last element read again (no branch around
+            // the loads), then a sentinel above every column
             int64_t xs[4];
             #pragma unroll
             for (int u = 0; u < 4; u++)
             {
                 const int64_t qq = q + u * step;
-                xs[u] = qq < q1 ? Sj[qq] : INT64_MAX;
+                xs[u] = Sj[qq < q1 ? qq : q1 - 1];
             }
+            #pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (q + u * step >= q1) xs[u] = INT64_MAX;
+            // the four probes' first index reads, issued together
+            uint64_t ws[4];
+            #pragma unroll
+            for (int u = 0; u < 4; u++) ws[u] = bm_first(bm, P, xs[u]);
             bool done = false;
             #pragma unroll
             for (int u = 0; u < 4; u++)
             {
                 const int64_t x = xs[u];
                 if (x > P.hi) { done = true; break; }     // also the end of the row
-                const int64_t pos = bm_find<!S::pattern>(bm, P, lp0, x);
+                const int64_t pos = bm_resolve<!S::pattern>(bm, P, lp0, x, ws[u]);
                 if (pos >= 0)
                 {
                     const int64_t qq = q + u * step;
@@ -951,6 +998,7 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_block_grp(Dot3Args g, const uint
 constexpr int64_t LC_POS_MAX = (int64_t) 1 << 28;   // chunk offsets kept (1 GB)
 constexpr int LC_WAVE_G = 16;                 // lanes per entry of the wavefront bin
 constexpr bool LC_NARROW = true;              // 32-bit short-row columns when they fit
+constexpr bool LC_TEAM16 = false;             // the 16-lane bin on the chunk kernel too
 constexpr int LC_LG = 20;
 constexpr int LC_BITS = 1 << LC_LG;           // columns per chunk: 128 KB of LDS
 constexpr int LC_WORDS = LC_BITS / 64;
@@ -1363,7 +1411,9 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
 // there: LDS reads in place of one random L2 / HBM access per element.  The
 // slice's other entries take the global path of the team kernel first (the
 // table's LDS doubles as their staging area then).  Slices are handed out
-// per XCD region as in the grouped team kernels.
+// per XCD region as in the grouped team kernels.  Seven workgroups share a
+// CU (LDS allows seven); registers are capped to match (72 VGPRs, a few
+// spilled: 15.1 against 16.4 ms at five waves per SIMD, RMAT 22).
 constexpr int LH_SLICE = 256;
 constexpr int LH_MAX = 2048;                   // long row entries at most
 constexpr int LH_SLOTS = 2 * LH_MAX;           // 16 KB of LDS
@@ -1373,10 +1423,11 @@ constexpr uint32_t LH_EMPTY = 0xffffffffu;
 static_assert(LH_SLOTS * sizeof(uint32_t) >= (D3_NT / 16) * StageCap<16>::value * sizeof(uint32_t),
               "the table's LDS holds the global path's staging lists");
 
-template <class S, int G>
-__global__ __launch_bounds__(D3_NT) void k_dot3_team_lh(Dot3Args g, const uint64_t *list,
+template <class S, int G, bool N32>
+__global__ __launch_bounds__(D3_NT) __attribute__((amdgpu_waves_per_eu(7))) void k_dot3_team_lh(Dot3Args g, const uint64_t *list,
                                                         const uint64_t *keys, int64_t count,
-                                                        unsigned long long *ctr)
+                                                        unsigned long long *ctr, const uint32_t *A32,
+                                                        const uint32_t *B32)
 {
     using Z = typename S::Z;
     static_assert(S::pattern, "LDS hash probing covers the pattern semirings");
@@ -1469,42 +1520,62 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_team_lh(Dot3Args g, const uint64
                     if (!P.empty && P.probe == (rowA ? 1 : 2))
                     {
                         // the short row from its first element in the long
-                        // row's range, four elements per lane per step, the
-                        // lane stopping after a step past the range
+                        // row's range, UN elements per lane per step (their
+                        // loads issued together; past the end: the last
+                        // element again), the lane stopping after a step
+                        // past the range
                         int64_t s0, s1;
                         d3_trim_short_team<G>(g, P, lane, s0, s1);
                         const int64_t *Sj = rowA ? g.Bj : g.Aj;
-                        for (int64_t qs = s0 + lane; qs < s1; qs += 4 * G)
+                        const uint32_t *Sj32 = rowA ? B32 : A32;   // N32: the same columns, 32-bit
+                        uint32_t cnt = 0;                          // plus_pair: this lane's matches
+                        constexpr int UN = N32 ? 8 : 4;
+                        for (int64_t qs = s0 + lane; qs < s1; qs += UN * G)
                         {
-                            int64_t xs[4];
+                            int64_t xs[UN];
                             #pragma unroll
-                            for (int u = 0; u < 4; u++)
+                            for (int u = 0; u < UN; u++)
                             {
                                 const int64_t qq = qs + u * G;
-                                xs[u] = qq < s1 ? (int64_t) Sj[qq] : INT64_MAX;
+                                const int64_t qc = qq < s1 ? qq : s1 - 1;
+                                if constexpr (N32) xs[u] = (int64_t) Sj32[qc];
+                                else xs[u] = Sj[qc];
                             }
                             #pragma unroll
-                            for (int u = 0; u < 4; u++)
+                            for (int u = 0; u < UN; u++)
+                                if (qs + u * G >= s1) xs[u] = INT64_MAX;
+                            #pragma unroll
+                            for (int u = 0; u < UN; u++)
                             {
                                 const int64_t x = xs[u];
                                 if (x > (int64_t) d.hi) break;
-                                uint32_t h = hx_hash((uint32_t) x, lg);
+                                const uint32_t x32 = (uint32_t) x;
+                                uint32_t h = hx_hash(x32, lg);
                                 bool hit = false;
                                 while (true)
                                 {
                                     const uint32_t v = s_tab[h];
-                                    if (v == (uint32_t) x) { hit = true; break; }
+                                    if (v == x32) { hit = true; break; }
                                     if (v == LH_EMPTY) break;
                                     h = (h + 1) & hmask;
                                 }
-                                if (!hit) continue;
-                                const Z tz = S::mul(Z(1), Z(1), x);
-                                if (!ex) { z = tz; ex = true; }
-                                else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
-                                else z = S::upd(z, tz);
+                                if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+                                {
+                                    cnt += hit ? 1u : 0u;
+                                }
+                                else
+                                {
+                                    if (!hit) continue;
+                                    const Z tz = S::mul(Z(1), Z(1), x);
+                                    if (!ex) { z = tz; ex = true; }
+                                    else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                                    else z = S::upd(z, tz);
+                                }
                             }
-                            if (xs[3] > (int64_t) d.hi) break;
+                            if (xs[UN - 1] > (int64_t) d.hi) break;
                         }
+                        if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+                            if (cnt) { ex = true; z = (Z) cnt; }
                     }
                     else if (!P.empty)
                     {
@@ -2140,7 +2211,7 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
     const int ncp = (int) ((A->ncols + LC_BITS - 1) >> LC_LG) + 1;
     const bool shared = (A->p == B->p && A->j == B->j && A->nrows == B->nrows);
     bool lds_chunk = false;
-    if (lds && (cnt[2] > 0 || (cnt[1] > 0 && !small)))
+    if (lds && (cnt[2] > 0 || (cnt[1] > 0 && !small) || (LC_TEAM16 && cnt[0] > 0 && !small)))
     {
         const int64_t na = A->nrows * ncp, nb = shared ? 0 : B->nrows * ncp;
         if (na + nb <= LC_POS_MAX && posA.alloc(sizeof(uint32_t) * (na > 0 ? na : 1)) &&
@@ -2161,7 +2232,7 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
     // the short rows' columns as 32-bit words halve the chunk kernel's reads
     DevBuf narA, narB;
     bool n32 = false;
-    if (lds_chunk && LC_NARROW && g.cols32)
+    if (lds && (lds_chunk || (cnt[0] > 0 && !small)) && LC_NARROW && g.cols32)
     {
         const int64_t nza = (int64_t) hc[4], nzb = shared ? 0 : (int64_t) hc[5];
         if (narA.alloc(sizeof(uint32_t) * (nza > 0 ? nza : 1)) &&
@@ -2200,6 +2271,22 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
                     g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
             GM_HIP(hipGetLastError(), "k_dot3_lds1");
         }
+        else if (b == 0 && lds_chunk && LC_TEAM16)
+        {
+            if constexpr (S::pattern)
+            {
+                ProfScope ps("dot3_team16_lds");
+                if (n32)
+                    k_dot3_lds<S, 16, true><<<(unsigned) cu_count(), LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>() + 16,
+                        pA, pB, ncp, nA, nB);
+                else
+                    k_dot3_lds<S, 16, false><<<(unsigned) cu_count(), LC_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>() + 16,
+                        pA, pB, ncp, nA, nB);
+                GM_HIP(hipGetLastError(), "k_dot3_lds");
+            }
+        }
         else if (b == 0 && lds)
         {
             // LDS hash tables of the long rows that pay for one (pattern
@@ -2207,8 +2294,12 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
             if constexpr (S::pattern)
             {
                 ProfScope ps("dot3_team16_lh");
-                k_dot3_team_lh<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
-                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
+                if (n32)
+                    k_dot3_team_lh<S, 16, true><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>(), nA, nB);
+                else
+                    k_dot3_team_lh<S, 16, false><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
+                        g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>(), nA, nB);
                 GM_HIP(hipGetLastError(), "k_dot3_team_lh");
             }
         }
