@@ -45,14 +45,14 @@ constexpr int D3_CT_TILE = D3_NT * D3_CT_IPT;
 // Probe indexes of the long rows of one operand (rows of length >= T).  A row
 // with column range [lo, hi] gets
 //   a bitmap  when its words are at most BM_DENSITY times its length
-//             (bm_eligible; 64 measured best on RMAT 22: one probe read
-//             instead of a probe sequence, and 8, 32, 128 and 512 all
-//             slower): ((hi-lo)>>6)+1 words, and rank[w] (valued
-//             semirings) counts the row's entries before word w;
+//             (bm_eligible; on RMAT 22, 32 builds the indexes 0.6 ms faster
+//             than 64 at the same probe times, and 16 probes 2.8 ms slower):
+//             ((hi-lo)>>6)+1 words, and rank[w] (valued semirings) counts
+//             the row's entries before word w;
 //   a hash    otherwise: 2^ceil(log2(2 len)) slots of (column << 32 | position),
 //             linear probing, EMPTY = all ones.
 // off[r] is the row's first 8-byte unit in the shared pool words[].
-constexpr int64_t BM_DENSITY = 64;
+constexpr int64_t BM_DENSITY = 32;
 constexpr uint64_t HX_EMPTY = ~0ull;
 
 struct RowBits

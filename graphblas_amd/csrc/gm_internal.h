@@ -297,6 +297,35 @@ template <class Z> __device__ __forceinline__ Z shfl_down(Z v, int off, int widt
     }
 }
 
+// lane src's value (index shuffle) for any payload of 1, 2, 4 or 8 bytes
+template <class Z> __device__ __forceinline__ Z shfl_any(Z v, int src, int width)
+{
+    if constexpr (sizeof(Z) == 8)
+    {
+        int64_t t;
+        __builtin_memcpy(&t, &v, 8);
+        int lo = __shfl((int) (t & 0xffffffff), src, width);
+        int hi = __shfl((int) (t >> 32), src, width);
+        int64_t r = ((int64_t) (uint32_t) lo) | ((int64_t) hi << 32);
+        Z out;
+        __builtin_memcpy(&out, &r, 8);
+        return out;
+    }
+    else if constexpr (sizeof(Z) == 4)
+    {
+        int t;
+        __builtin_memcpy(&t, &v, 4);
+        t = __shfl(t, src, width);
+        Z out;
+        __builtin_memcpy(&out, &t, 4);
+        return out;
+    }
+    else
+    {
+        return (Z) __shfl((int) v, src, width);
+    }
+}
+
 // in-order tree reduction over a team of G lanes (G a power of 2 <= 64);
 // lane 0 of the team holds the result
 template <class S, int G>

@@ -789,11 +789,6 @@ void k_sx_fused(SxArgs g, int64_t fs,
     // start on different LDS banks
     __shared__ K s_key[FUSED_ROWS][FMAX + 1];         // sort keys, then output columns (int32)
     __shared__ Z s_val[FUSED_ROWS][FMAX + 1];         // flop values, then output values
-    __shared__ int32_t s_pre[FUSED_ROWS][TG + 1];
-    // the inner index k of each staged A entry: only positional multipliers use it
-    constexpr int KROWS = (S::mult == GM_MULT_K) ? FUSED_ROWS : 1;
-    __shared__ int64_t s_k[KROWS][TG + 1], s_bs[FUSED_ROWS][TG + 1];
-    __shared__ Z s_av[FUSED_ROWS][TG + 1];
     __shared__ int64_t s_cnt[FUSED_ROWS], s_off[FUSED_ROWS + 1];
     __shared__ int s_big[FUSED_ROWS];
     __shared__ int64_t s_tile, s_prefix;
@@ -846,25 +841,30 @@ void k_sx_fused(SxArgs g, int64_t fs,
                 const int64_t inc = team_incl_scan<int64_t>(len);
                 const int64_t tot = __shfl(inc, TG - 1, TG);
                 if (f + tot > fs) { big = 1; break; }
-                s_pre[team][sub] = (int32_t) (inc - len);
-                if constexpr (S::mult == GM_MULT_K) s_k[team][sub] = k;
-                s_bs[team][sub] = bs;
-                s_av[team][sub] = av;
-                const int nA = (a1 - e0) < TG ? (int) (a1 - e0) : TG;
-                wave_sync();
+                // the chunk's flop offsets in every lane's registers; flop q
+                // belongs to the last A entry whose offset is <= q (entries
+                // past the row's end have offset tot, never <= q)
+                const int32_t pre = (int32_t) (inc - len);
+                int32_t pres[TG];
+                #pragma unroll
+                for (int m = 0; m < TG; m++) pres[m] = __shfl(pre, m, TG);
                 #pragma unroll
                 for (int it = 0; it < FMAX / TG; it++)
                 {
                     const int64_t q = (int64_t) it * TG + sub;
+                    int lo = -1;
+                    #pragma unroll
+                    for (int m = 0; m < TG; m++) lo += (pres[m] <= (int32_t) q) ? 1 : 0;
+                    lo = lo < 0 ? 0 : lo;
+                    // the owner's fields, shuffled by every lane of the team (a
+                    // lane outside the branch below would return zeros)
+                    const int64_t bsl = __shfl(bs, lo, TG);
+                    const int32_t prl = __shfl(pre, lo, TG);
+                    const Z avl = shfl_any<Z>(av, lo, TG);
+                    const int64_t kl = (S::mult == GM_MULT_K) ? __shfl(k, lo, TG) : 0;
                     if (q < tot)
                     {
-                        int lo = 0, hi = nA - 1;
-                        while (lo < hi)
-                        {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (s_pre[team][mid] <= q) lo = mid; else hi = mid - 1;
-                        }
-                        const int64_t pb = s_bs[team][lo] + (q - s_pre[team][lo]);
+                        const int64_t pb = bsl + (q - prl);
                         int64_t j;
                         Z bv;
                         if constexpr (PK != 0)
@@ -882,8 +882,7 @@ void k_sx_fused(SxArgs g, int64_t fs,
                             if constexpr (!S::iso_out)
                             {
                                 if constexpr (PK == 0) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
-                                val[fq] = S::mul(s_av[team][lo], bv,
-                                                 (S::mult == GM_MULT_K) ? s_k[KROWS > 1 ? team : 0][lo] : 0);
+                                val[fq] = S::mul(avl, bv, kl);
                             }
                             nallow++;
                         }
@@ -948,8 +947,19 @@ void k_sx_fused(SxArgs g, int64_t fs,
             #pragma unroll
             for (int r = 0; r < PER_L; r++) key[sub * PER_L + r] = kr[r];
             wave_sync();
-            // compress: lane owns sorted positions sub*8 .. sub*8+7
+            // compress: lane owns sorted positions sub*8 .. sub*8+7, whose
+            // keys are still in kr[]; a run's head sums its values in flop
+            // order from registers, and from LDS only past the lane's end
             constexpr int PER = FMAX / TG;
+            const K kprev = (K) shfl_any<K>(kr[PER - 1], sub > 0 ? sub - 1 : 0, TG);
+            Z vv[PER];
+            #pragma unroll
+            for (int u = 0; u < PER; u++)
+            {
+                vv[u] = S::identity();
+                if constexpr (!S::iso_out)
+                    if (sub * PER + u < nallow) vv[u] = val[kr[u] & 63];
+            }
             int32_t hj[PER];
             Z hz[PER];
             int nh = 0;
@@ -960,19 +970,28 @@ void k_sx_fused(SxArgs g, int64_t fs,
                 hj[u] = -1;
                 if (p < nallow)
                 {
-                    const K kp = key[p];
-                    const int64_t jp = (int64_t) (kp >> 6);
-                    if (p == 0 || (int64_t) (key[p - 1] >> 6) != jp)
+                    const int64_t jp = (int64_t) (kr[u] >> 6);
+                    const K kb = (u == 0) ? kprev : kr[u > 0 ? u - 1 : 0];
+                    if (p == 0 || (int64_t) (kb >> 6) != jp)
                     {
-                        Z z = S::identity();
-                        if constexpr (!S::iso_out)
+                        Z z = vv[u];
+                        bool open = true;
+                        #pragma unroll
+                        for (int u2 = u + 1; u2 < PER; u2++)
                         {
-                            z = val[kp & 63];
-                            for (int p2 = p + 1; p2 < nallow; p2++)
+                            if (open && sub * PER + u2 < nallow && (int64_t) (kr[u2] >> 6) == jp)
+                            {
+                                if constexpr (!S::iso_out) z = S::upd(z, vv[u2]);
+                            }
+                            else open = false;
+                        }
+                        if (open)
+                        {
+                            for (int p2 = sub * PER + PER; p2 < nallow; p2++)
                             {
                                 const K k2 = key[p2];
                                 if ((int64_t) (k2 >> 6) != jp) break;
-                                z = S::upd(z, val[k2 & 63]);
+                                if constexpr (!S::iso_out) z = S::upd(z, val[k2 & 63]);
                             }
                         }
                         hj[u] = (int32_t) jp;
