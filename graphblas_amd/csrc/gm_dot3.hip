@@ -296,6 +296,8 @@ __device__ __forceinline__ int64_t bm_resolve(const RowBits &bm, const Pair &P, 
 // search; steps of four elements keep their loads and probes issued
 // together (team16 bin at RMAT 22: 25.9 ms with an exit per element, 23.1
 // ms in steps of four).  Without STOP the loop has no data-dependent exit.
+constexpr int PR_UN = 4;                      // elements per lane per probe step
+constexpr int MERGE_BLK = 4;                  // block width of the serial merge
 template <class S, bool STOP>
 __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P, int64_t q0, int64_t q1,
                                                int64_t step, bool &e, typename S::Z &z)
@@ -309,29 +311,29 @@ __device__ __forceinline__ void d3_probe_range(const Dot3Args &g, const Pair &P,
     const int64_t lp0 = shortA ? P.pb : P.pa;
     if constexpr (STOP)
     {
-        // four elements per step: their loads, then their probes, issue
+        // PR_UN elements per step: their loads, then their probes, issue
         // together; the lane stops after a step that passed the range
-        for (int64_t q = q0; q < q1; q += 4 * step)
+        for (int64_t q = q0; q < q1; q += PR_UN * step)
         {
             // past the end: the last element read again (no branch around
             // the loads), then a sentinel above every column
-            int64_t xs[4];
+            int64_t xs[PR_UN];
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < PR_UN; u++)
             {
                 const int64_t qq = q + u * step;
                 xs[u] = Sj[qq < q1 ? qq : q1 - 1];
             }
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < PR_UN; u++)
                 if (q + u * step >= q1) xs[u] = INT64_MAX;
-            // the four probes' first index reads, issued together
-            uint64_t ws[4];
+            // the probes' first index reads, issued together
+            uint64_t ws[PR_UN];
             #pragma unroll
-            for (int u = 0; u < 4; u++) ws[u] = bm_first(bm, P, xs[u]);
+            for (int u = 0; u < PR_UN; u++) ws[u] = bm_first(bm, P, xs[u]);
             bool done = false;
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < PR_UN; u++)
             {
                 const int64_t x = xs[u];
                 if (x > P.hi) { done = true; break; }     // also the end of the row
@@ -456,23 +458,24 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
     const bool trimA = a > 8 * b, trimB = b > 8 * a;
     if (!trimA && !trimB)
     {
-        // blocks of four from each row, their loads issued together; all
+        // blocks of MERGE_BLK from each row, their loads issued together; all
         // pairs of a block compared in registers, then the block with the
         // smaller last column (or both) moves on.  Matches come in
         // ascending column order, as in the element-wise merge.
         const int64_t ahi = P.lo, bhi = P.hi;
+        constexpr int MB = MERGE_BLK;
         while (pa < pa_end && pb < pb_end)
         {
-            int64_t xa[4], xb[4];
+            int64_t xa[MB], xb[MB];
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < MB; u++)
             {
                 xa[u] = Aj[pa + u < pa_end ? pa + u : pa_end - 1];
                 xb[u] = Bj[pb + u < pb_end ? pb + u : pb_end - 1];
             }
             // past a row's end: sentinels that match nothing
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < MB; u++)
             {
                 if (pa + u >= pa_end) xa[u] = INT64_MAX;
                 if (pb + u >= pb_end) xb[u] = INT64_MAX - 1;
@@ -480,10 +483,10 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
             if (xa[0] > bhi || xb[0] > ahi) break;
             bool stop = false;
             #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int u = 0; u < MB; u++)
             {
                 #pragma unroll
-                for (int v = 0; v < 4; v++)
+                for (int v = 0; v < MB; v++)
                 {
                     if (xa[u] == xb[v] && !stop)
                     {
@@ -495,9 +498,9 @@ __device__ bool dot_serial(const Dot3Args &g, const Pair &P, typename S::Z &out)
                 }
             }
             if (stop) break;
-            const int64_t am = xa[3], bm = xb[3];
-            if (am <= bm) pa += 4;
-            if (bm <= am) pb += 4;
+            const int64_t am = xa[MB - 1], bm = xb[MB - 1];
+            if (am <= bm) pa += MB;
+            if (bm <= am) pb += MB;
         }
         if constexpr (S::plus_pair_wide) e = (z != 0);
         out = z;
@@ -749,6 +752,13 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
     __shared__ int64_t s_rows[2];
     // the tile's row pointers Mp[r0 .. r1+1], when the window is short
     __shared__ int64_t s_mp[D3_P1_TILE + 2];
+    // the tile's binned entries: bin (-1: none), record, grouping key; the
+    // wavefronts' ballots per bin, their offsets in the tile, the bases
+    __shared__ int8_t s_bin[D3_P1_TILE];
+    __shared__ uint64_t s_rec[D3_P1_TILE], s_bkey[D3_P1_TILE];
+    __shared__ unsigned long long s_mb[D3_P1_PER_THREAD][D3_NT / 64][3];
+    __shared__ unsigned long long s_pre[D3_P1_PER_THREAD][D3_NT / 64][3];
+    __shared__ unsigned long long s_base[3];
     const int64_t ntiles = (g.mnz + D3_P1_TILE - 1) / D3_P1_TILE;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
     {
@@ -823,17 +833,50 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
                     if (e && !S::iso_out) ((Z *) g.W)[p] = z;
                 }
             }
+            // the entry's bin and record, kept in LDS for the tile's appends;
+            // lane 0 keeps the wavefront's ballot of each bin
+            const int tslot = u * D3_NT + (int) threadIdx.x;
+            s_bin[tslot] = (int8_t) (bin >= 0 ? bin : -1);
+            if (bin >= 0)
+            {
+                s_rec[tslot] = ((uint64_t) i << 32) | (uint64_t) p;
+                s_bkey[tslot] = key;
+            }
             #pragma unroll
             for (int b = 0; b < 3; b++)
             {
-                const bool take = (bin == b);
-                const unsigned long long slot = wave_append(take, &g.bin_cnt[b]);
-                if (take)
-                {
-                    g.bins[(int64_t) b * g.mnz + (int64_t) slot] = ((uint64_t) i << 32) | (uint64_t) p;
-                    if (g.bkeys) g.bkeys[(int64_t) b * g.mnz + (int64_t) slot] = key;
-                }
+                const unsigned long long m = __ballot(bin == b);
+                if ((threadIdx.x & 63) == 0) s_mb[u][threadIdx.x >> 6][b] = m;
             }
+        }
+        // one global atomic per bin per tile: every wavefront appending to
+        // three shared counters serialised phase1 on them (RMAT 22: 17 of
+        // its 19 ms)
+        __syncthreads();
+        if (threadIdx.x < 3)
+        {
+            const int b = (int) threadIdx.x;
+            unsigned long long run = 0;
+            for (int u = 0; u < D3_P1_PER_THREAD; u++)
+                for (int w = 0; w < D3_NT / 64; w++)
+                {
+                    s_pre[u][w][b] = run;
+                    run += (unsigned long long) __popcll(s_mb[u][w][b]);
+                }
+            s_base[b] = run ? atomicAdd(&g.bin_cnt[b], run) : 0ull;
+        }
+        __syncthreads();
+        #pragma unroll
+        for (int u = 0; u < D3_P1_PER_THREAD; u++)
+        {
+            const int tslot = u * D3_NT + (int) threadIdx.x;
+            const int b = s_bin[tslot];
+            if (b < 0) continue;
+            const int wv = (int) (threadIdx.x >> 6), wl = (int) (threadIdx.x & 63);
+            const unsigned long long below = s_mb[u][wv][b] & ((1ull << wl) - 1ull);
+            const int64_t slot = (int64_t) (s_base[b] + s_pre[u][wv][b] + (unsigned long long) __popcll(below));
+            g.bins[(int64_t) b * g.mnz + slot] = s_rec[tslot];
+            if (g.bkeys) g.bkeys[(int64_t) b * g.mnz + slot] = s_bkey[tslot];
         }
         __syncthreads();
     }
@@ -1264,10 +1307,11 @@ constexpr int L1_WORDS = L1_BITS / 64;
 constexpr int L1_SLICE = 256;
 constexpr int L1_GMIN = 4;
 
-template <class S, int G>
+template <class S, int G, bool N32>
 __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t *list,
                                                      const uint64_t *keys, int64_t count,
-                                                     unsigned long long *ctr)
+                                                     unsigned long long *ctr, const uint32_t *A32,
+                                                     const uint32_t *B32)
 {
     using Z = typename S::Z;
     constexpr int TPB = D3_NT / G;
@@ -1350,42 +1394,64 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
                     const int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
                     bool e = false;
                     Z z = S::identity();
-                    // steps of four elements per lane, loads issued together;
-                    // the lane stops after a step past the long row's range
-                    for (int64_t q0s = s0 + lane; q0s < s1; q0s += 4 * G)
+                    uint32_t cnt = 0;                     // plus_pair: this lane's matches
+                    // steps of UN elements per lane, loads issued together (past
+                    // the end: the last element again, no branch around the
+                    // loads); the lane stops after a step past the long row's range
+                    constexpr int UN = N32 ? 8 : 4;
+                    const uint32_t *Sj32 = shortA ? A32 : B32;
+                    for (int64_t q0s = s0 + lane; q0s < s1; q0s += UN * G)
                     {
-                        int64_t xs[4];
+                        int64_t xs[UN];
                         #pragma unroll
-                        for (int u = 0; u < 4; u++)
+                        for (int u = 0; u < UN; u++)
                         {
                             const int64_t qq = q0s + u * G;
-                            xs[u] = qq < s1 ? Sj[qq] : INT64_MAX;
+                            const int64_t qc = qq < s1 ? qq : s1 - 1;
+                            if constexpr (N32) xs[u] = (int64_t) Sj32[qc];
+                            else xs[u] = Sj[qc];
                         }
                         #pragma unroll
-                        for (int u = 0; u < 4; u++)
+                        for (int u = 0; u < UN; u++)
+                            if (q0s + u * G >= s1) xs[u] = INT64_MAX;
+                        #pragma unroll
+                        for (int u = 0; u < UN; u++)
                         {
                             const int64_t x = xs[u];
                             if (x < (int64_t) d.lo || x > (int64_t) d.hi) continue;
-                            const int64_t dd = x - d.lo;
-                            const uint64_t word = s_bits[dd >> 6];
-                            const uint64_t bit = 1ull << (dd & 63);
-                            if (!(word & bit)) continue;
-                            Z tz;
+                            const uint32_t dd = (uint32_t) (x - d.lo);
                             if constexpr (S::pattern)
-                                tz = S::mul(Z(1), Z(1), x);
+                            {
+                                const uint32_t hit = (((const uint32_t *) s_bits)[dd >> 5] >> (dd & 31)) & 1u;
+                                if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+                                {
+                                    cnt += hit;
+                                }
+                                else if (hit)
+                                {
+                                    const Z tz = S::mul(Z(1), Z(1), x);
+                                    if (!e) { z = tz; e = true; }
+                                    else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                                    else z = S::upd(z, tz);
+                                }
+                            }
                             else
                             {
+                                const uint64_t word = s_bits[dd >> 6];
+                                const uint64_t bit = 1ull << (dd & 63);
+                                if (!(word & bit)) continue;
                                 const int64_t q = q0s + u * G;
                                 const int64_t pos = d.p0 + (int64_t) s_rank[dd >> 6] + __popcll(word & (bit - 1));
                                 const int64_t qa = shortA ? q : pos, qb = shortA ? pos : q;
-                                tz = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                                const Z tz = S::mul(S::geta(Ax, qa, g.a_iso), S::geta(Bx, qb, g.b_iso), x);
+                                if (!e) { z = tz; e = true; }
+                                else z = S::upd(z, tz);
                             }
-                            if (!e) { z = tz; e = true; }
-                            else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
-                            else z = S::upd(z, tz);
                         }
-                        if (xs[3] > (int64_t) d.hi) break;
+                        if (xs[UN - 1] > (int64_t) d.hi) break;
                     }
+                    if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+                        if (cnt) { e = true; z = (Z) cnt; }
                     team_reduce<S, G>(e, z);
                     if (lane == 0) d3_store<S>(g, p, e, z);
                 }
@@ -2232,7 +2298,8 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
     // the short rows' columns as 32-bit words halve the chunk kernel's reads
     DevBuf narA, narB;
     bool n32 = false;
-    if (lds && (lds_chunk || (cnt[0] > 0 && !small)) && LC_NARROW && g.cols32)
+    if (LC_NARROW && g.cols32 && g.dA != nullptr &&
+        ((small && (cnt[0] > 0 || cnt[1] > 0)) || (lds && (lds_chunk || cnt[0] > 0))))
     {
         const int64_t nza = (int64_t) hc[4], nzb = shared ? 0 : (int64_t) hc[5];
         if (narA.alloc(sizeof(uint32_t) * (nza > 0 ? nza : 1)) &&
@@ -2263,12 +2330,20 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
         if (b < 2 && small)
         {
             ProfScope ps(b == 0 ? "dot3_team16_lds1" : "dot3_wave_lds1");
-            if (b == 0)
-                k_dot3_lds1<S, 16><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
-                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[0], ctr.as<unsigned long long>());
+            unsigned long long *c = ctr.as<unsigned long long>() + (b == 0 ? 0 : 8);
+            const unsigned grid = (unsigned) (cu_count() * 8);
+            if (b == 0 && n32)
+                k_dot3_lds1<S, 16, true><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
+                                                                  cnt[0], c, nA, nB);
+            else if (b == 0)
+                k_dot3_lds1<S, 16, false><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
+                                                                   cnt[0], c, nA, nB);
+            else if (n32)
+                k_dot3_lds1<S, 64, true><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
+                                                                  cnt[1], c, nA, nB);
             else
-                k_dot3_lds1<S, 64><<<(unsigned) (cu_count() * 8), D3_NT, 0, st>>>(
-                    g, tv.as<uint64_t>(), tk.as<uint64_t>(), cnt[1], ctr.as<unsigned long long>() + 8);
+                k_dot3_lds1<S, 64, false><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
+                                                                   cnt[1], c, nA, nB);
             GM_HIP(hipGetLastError(), "k_dot3_lds1");
         }
         else if (b == 0 && lds_chunk && LC_TEAM16)
@@ -2406,7 +2481,7 @@ static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_c
                 return set_error(GM_OUT_OF_MEMORY, "dot3: grouping keys");
             g.bkeys = bkeys.as<uint64_t>();
         }
-        g.thread_max = pick_param(prm ? prm->thread_max : 0, tu.thread_max, 64);
+        g.thread_max = pick_param(prm ? prm->thread_max : 0, tu.thread_max, 8);
         g.team16_max = pick_param(prm ? prm->team16_max : 0, tu.team16_max, 512);
         g.wave_max = pick_param(prm ? prm->wave_max : 0, tu.wave_max, 4096);
         const int64_t cap = (int64_t) cu_count() * 16;

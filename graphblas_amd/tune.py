@@ -25,14 +25,14 @@ from . import capi
 from . import workloads as W
 
 DOT3_SPACE = {
-    "thread_max": [8, 16, 32, 64],
+    "thread_max": [4, 8, 16, 32, 64],
     "team16_max": [128, 256, 512, 1024],
     "wave_max": [2048, 4096, 8192],
     "bitmap_min_len": [64, 128, 256, 512, -1],
     "order": [1, 2],
     "row_min": [16, 64, 256, 1024],
 }
-DOT3_BUILTIN = {"thread_max": 64, "team16_max": 512, "wave_max": 4096, "bitmap_min_len": 128,
+DOT3_BUILTIN = {"thread_max": 8, "team16_max": 512, "wave_max": 4096, "bitmap_min_len": 128,
                 "order": 0, "row_min": -1}
 SAXPY3_SPACE = {
     "wave_small_max": [16, 32, 64],

@@ -176,7 +176,7 @@ void gm_csr_aux_free(gm_csr_aux *aux);
 // min(|A(i,:)|+|B(j,:)|, s*ceil(log2(l+1))) with s,l the short/long list.
 typedef struct
 {
-    int64_t thread_max;   // work <= thread_max: one thread per entry (64)
+    int64_t thread_max;   // work <= thread_max: one thread per entry (8)
     int64_t team16_max;   // work <= team16_max: 16 lanes per entry (512)
     int64_t wave_max;     // work <= wave_max: one wavefront per entry (4096)
                           // larger: one 256-thread workgroup per entry
