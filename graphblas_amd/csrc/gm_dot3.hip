@@ -886,7 +886,8 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_phase1(Dot3Args g)
 // team kernels: G lanes per deferred entry (G = 16 or 64)
 //------------------------------------------------------------------------------
 
-// one deferred entry w = (i << 32 | p), computed by a team of G lanes
+// one deferred entry w = (i << 32 | p), computed by a team of G lanes (sl:
+// the team's LDS staging area for short merges, or NULL for none)
 template <class S, int G>
 __device__ __forceinline__ void team_task(const Dot3Args &g, uint64_t w, int lane, uint32_t *sl)
 {
@@ -898,7 +899,7 @@ __device__ __forceinline__ void team_task(const Dot3Args &g, uint64_t w, int lan
     Z z;
     Pair P;
     (void) d3_pair(g, i, j, P);
-    if (!P.empty && P.probe == 0 && g.cols32 &&
+    if (sl && !P.empty && P.probe == 0 && g.cols32 &&
         (P.pa_end - P.pa) + (P.pb_end - P.pb) <= CAP)
         dot_lane_staged<S, G>(g, P, lane, sl, e, z);
     else
@@ -1320,6 +1321,12 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
     __shared__ uint32_t s_rank[RANK ? L1_WORDS : 1];
     __shared__ uint32_t s_lists[TPB][StageCap<G>::value];
     __shared__ uint64_t s_key[L1_SLICE];
+    // every slice entry's short row (start, length) and probe direction
+    // (0: none, 1: into A's row, 2: into B's row), set up together at the
+    // slice's start so that a team's entry starts with its row's loads
+    __shared__ int64_t s_sp0[L1_SLICE];
+    __shared__ uint32_t s_slen[L1_SLICE];
+    __shared__ int8_t s_pdir[L1_SLICE];
     __shared__ int64_t s_slice;
     __shared__ int s_ge;
     const int lane = threadIdx.x & (G - 1), team = threadIdx.x / G;
@@ -1341,7 +1348,26 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
             if (sl_id >= shi) break;
             const int64_t q0 = sl_id * L1_SLICE;
             const int n = (int) ((count - q0) < L1_SLICE ? (count - q0) : L1_SLICE);
-            for (int t = threadIdx.x; t < L1_SLICE; t += D3_NT) s_key[t] = t < n ? keys[q0 + t] : ~0ull;
+            for (int t = threadIdx.x; t < L1_SLICE; t += D3_NT)
+            {
+                s_key[t] = t < n ? keys[q0 + t] : ~0ull;
+                int8_t dir = 0;
+                if (t < n)
+                {
+                    const uint64_t w = list[q0 + t];
+                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
+                    Pair P;
+                    (void) d3_pair(g, i, g.Mj[p], P);
+                    if (!P.empty && P.probe != 0)
+                    {
+                        dir = (int8_t) P.probe;
+                        const bool shortA = (P.probe == 2);
+                        s_sp0[t] = shortA ? P.pa : P.pb;
+                        s_slen[t] = (uint32_t) (shortA ? P.pa_end - P.pa : P.pb_end - P.pb);
+                    }
+                }
+                s_pdir[t] = dir;
+            }
             __syncthreads();
             int gs = 0;
             while (gs < n)
@@ -1379,19 +1405,16 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
                 }
                 for (int t = gs + team; t < ge; t += TPB)
                 {
-                    const uint64_t w = list[q0 + t];
-                    const int64_t i = (int64_t) (w >> 32), p = (int64_t) (w & 0xffffffffull);
-                    Pair P;
-                    (void) d3_pair(g, i, g.Mj[p], P);
-                    if (!(use_lds && !P.empty && P.probe == (rowA ? 1 : 2)))
+                    if (!(use_lds && s_pdir[t] == (rowA ? 1 : 2)))
                     {
-                        team_task<S, G>(g, w, lane, sl);
+                        team_task<S, G>(g, list[q0 + t], lane, sl);
                         continue;
                     }
+                    const int64_t p = (int64_t) (list[q0 + t] & 0xffffffffull);
                     // the short row against the LDS bitmap of the long one
                     const bool shortA = !rowA;
                     const int64_t *Sj = shortA ? g.Aj : g.Bj;
-                    const int64_t s0 = shortA ? P.pa : P.pb, s1 = shortA ? P.pa_end : P.pb_end;
+                    const int64_t s0 = s_sp0[t], s1 = s0 + (int64_t) s_slen[t];
                     bool e = false;
                     Z z = S::identity();
                     uint32_t cnt = 0;                     // plus_pair: this lane's matches
@@ -1488,6 +1511,62 @@ constexpr int LH_PAY = 32;
 constexpr uint32_t LH_EMPTY = 0xffffffffu;
 static_assert(LH_SLOTS * sizeof(uint32_t) >= (D3_NT / 16) * StageCap<16>::value * sizeof(uint32_t),
               "the table's LDS holds the global path's staging lists");
+
+// One lane's share of a short row [s0, s1) probed against an LDS hash table
+// of the long row's 32-bit columns (2^lg slots, linear probing): UN elements
+// per lane per step, their loads issued together (past the end: the last
+// element again); the lane stops after a step past the long row's last column.
+template <class S, int G, bool N32, int UN>
+__device__ __forceinline__ void lh_probe(const int64_t *Sj, const uint32_t *Sj32, int64_t s0, int64_t s1,
+                                         int lane, int64_t hi, const uint32_t *tab, int lg,
+                                         uint32_t hmask, bool &ex, typename S::Z &z, uint32_t &cnt)
+{
+    using Z = typename S::Z;
+    for (int64_t qs = s0 + lane; qs < s1; qs += UN * G)
+    {
+        int64_t xs[UN];
+        #pragma unroll
+        for (int u = 0; u < UN; u++)
+        {
+            const int64_t qq = qs + u * G;
+            const int64_t qc = qq < s1 ? qq : s1 - 1;
+            if constexpr (N32) xs[u] = (int64_t) Sj32[qc];
+            else xs[u] = Sj[qc];
+        }
+        #pragma unroll
+        for (int u = 0; u < UN; u++)
+            if (qs + u * G >= s1) xs[u] = INT64_MAX;
+        #pragma unroll
+        for (int u = 0; u < UN; u++)
+        {
+            const int64_t x = xs[u];
+            if (x > hi) break;
+            const uint32_t x32 = (uint32_t) x;
+            uint32_t h = hx_hash(x32, lg);
+            bool hit = false;
+            while (true)
+            {
+                const uint32_t v = tab[h];
+                if (v == x32) { hit = true; break; }
+                if (v == LH_EMPTY) break;
+                h = (h + 1) & hmask;
+            }
+            if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
+            {
+                cnt += hit ? 1u : 0u;
+            }
+            else
+            {
+                if (!hit) continue;
+                const Z tz = S::mul(Z(1), Z(1), x);
+                if (!ex) { z = tz; ex = true; }
+                else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
+                else z = S::upd(z, tz);
+            }
+        }
+        if (xs[UN - 1] > hi) break;
+    }
+}
 
 template <class S, int G, bool N32>
 __global__ __launch_bounds__(D3_NT) __attribute__((amdgpu_waves_per_eu(7))) void k_dot3_team_lh(Dot3Args g, const uint64_t *list,
@@ -1586,60 +1665,13 @@ __global__ __launch_bounds__(D3_NT) __attribute__((amdgpu_waves_per_eu(7))) void
                     if (!P.empty && P.probe == (rowA ? 1 : 2))
                     {
                         // the short row from its first element in the long
-                        // row's range, UN elements per lane per step (their
-                        // loads issued together; past the end: the last
-                        // element again), the lane stopping after a step
-                        // past the range
+                        // row's range, probed in the LDS table (lh_probe)
                         int64_t s0, s1;
                         d3_trim_short_team<G>(g, P, lane, s0, s1);
                         const int64_t *Sj = rowA ? g.Bj : g.Aj;
                         const uint32_t *Sj32 = rowA ? B32 : A32;   // N32: the same columns, 32-bit
                         uint32_t cnt = 0;                          // plus_pair: this lane's matches
-                        constexpr int UN = N32 ? 8 : 4;
-                        for (int64_t qs = s0 + lane; qs < s1; qs += UN * G)
-                        {
-                            int64_t xs[UN];
-                            #pragma unroll
-                            for (int u = 0; u < UN; u++)
-                            {
-                                const int64_t qq = qs + u * G;
-                                const int64_t qc = qq < s1 ? qq : s1 - 1;
-                                if constexpr (N32) xs[u] = (int64_t) Sj32[qc];
-                                else xs[u] = Sj[qc];
-                            }
-                            #pragma unroll
-                            for (int u = 0; u < UN; u++)
-                                if (qs + u * G >= s1) xs[u] = INT64_MAX;
-                            #pragma unroll
-                            for (int u = 0; u < UN; u++)
-                            {
-                                const int64_t x = xs[u];
-                                if (x > (int64_t) d.hi) break;
-                                const uint32_t x32 = (uint32_t) x;
-                                uint32_t h = hx_hash(x32, lg);
-                                bool hit = false;
-                                while (true)
-                                {
-                                    const uint32_t v = s_tab[h];
-                                    if (v == x32) { hit = true; break; }
-                                    if (v == LH_EMPTY) break;
-                                    h = (h + 1) & hmask;
-                                }
-                                if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
-                                {
-                                    cnt += hit ? 1u : 0u;
-                                }
-                                else
-                                {
-                                    if (!hit) continue;
-                                    const Z tz = S::mul(Z(1), Z(1), x);
-                                    if (!ex) { z = tz; ex = true; }
-                                    else if constexpr (S::any_min_k) { if (tz < z) z = tz; }
-                                    else z = S::upd(z, tz);
-                                }
-                            }
-                            if (xs[UN - 1] > (int64_t) d.hi) break;
-                        }
+                        lh_probe<S, G, N32, N32 ? 8 : 4>(Sj, Sj32, s0, s1, lane, d.hi, s_tab, lg, hmask, ex, z, cnt);
                         if constexpr (S::add == GM_ADD_PLUS && S::mult == GM_MULT_PAIR)
                             if (cnt) { ex = true; z = (Z) cnt; }
                     }
@@ -2334,10 +2366,10 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
             const unsigned grid = (unsigned) (cu_count() * 8);
             if (b == 0 && n32)
                 k_dot3_lds1<S, 16, true><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
-                                                                  cnt[0], c, nA, nB);
+                                                                   cnt[0], c, nA, nB);
             else if (b == 0)
                 k_dot3_lds1<S, 16, false><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
-                                                                   cnt[0], c, nA, nB);
+                                                                    cnt[0], c, nA, nB);
             else if (n32)
                 k_dot3_lds1<S, 64, true><<<grid, D3_NT, 0, st>>>(g, tv.as<uint64_t>(), tk.as<uint64_t>(),
                                                                   cnt[1], c, nA, nB);
