@@ -702,7 +702,7 @@ __global__ void k_sx_iso_one(void *x, int size)
 // single-pass fused kernel for small rows (f <= fs <= 64)
 //------------------------------------------------------------------------------
 //
-// A persistent grid takes tiles of SX_NT/TG = 32 consecutive rows in ticket
+// A persistent grid takes tiles of FUSED_NT/TG = 64 consecutive rows in ticket
 // order.  One team of TG = 8 lanes per row:
 //   expand   the row's flops, in the reference's order (A(i,:) in chunks of
 //            TG entries, B(k,:) in order), into LDS as sort keys
@@ -760,7 +760,9 @@ __global__ __launch_bounds__(256) void k_sx_rec_build(const int64_t *__restrict_
     }
 }
 
-constexpr int FUSED_NT = 256;              // threads of a fused-kernel workgroup
+// threads of a fused-kernel workgroup: 64-row tiles halve the look-backs of
+// 32-row ones (ER 2^20: 0.861 against 0.899 ms; 16-row tiles 1.32 ms)
+constexpr int FUSED_NT = 512;
 constexpr int FUSED_ROWS = FUSED_NT / TG;  // rows of a tile
 constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
@@ -770,14 +772,13 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
-// Five waves per SIMD (<= 102 VGPRs, LDS <= 32 KB) with 32-bit keys: the
-// look-back of one workgroup is hidden behind the other workgroups of the
-// CU (measured: 2% faster than four waves even with a few spilled
-// registers).  Positional multipliers keep four (their k array needs the
-// LDS); 64-bit keys double the key LDS, which then allows three.
+// With 32-bit keys the bound asks for five waves per SIMD (<= 102 VGPRs):
+// the look-back of one workgroup is hidden behind the other workgroups of
+// the CU (measured: 2% faster than four waves even with a few spilled
+// registers); 64-bit keys double the key LDS and ask for three.
 // PK: 0 B read from Bp/Bj/Bx, 1 from the packed pairs, 2 from the row records
 template <class S, class K, int NT, int PK>
-__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? (S::mult == GM_MULT_K ? 1024 : 1280) : 768) / NT)
+__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1280 : 768) / NT)
 void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
                                                          int *err)
@@ -1381,7 +1382,7 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
         }
         {
             ProfScope ps("sx_fused");
-            const int64_t gcap = (int64_t) cu_count() * 8 * (256 / FUSED_NT);
+            const int64_t gcap = (int64_t) cu_count() * 8 * 256 / FUSED_NT;
             const int64_t grid = ntiles < gcap ? ntiles : gcap;
             unsigned long long *stp = status.as<unsigned long long>();
             int *errp = (int *) (g.cnt + 12);
