@@ -763,6 +763,7 @@ __global__ __launch_bounds__(256) void k_sx_rec_build(const int64_t *__restrict_
 // threads of a fused-kernel workgroup: 64-row tiles halve the look-backs of
 // 32-row ones (ER 2^20: 0.861 against 0.899 ms; 16-row tiles 1.32 ms)
 constexpr int FUSED_NT = 512;
+constexpr int SX_FUSED_WAVES = 4;
 constexpr int FUSED_ROWS = FUSED_NT / TG;  // rows of a tile
 constexpr unsigned long long LB_AGG = 1ull << 62;
 constexpr unsigned long long LB_PRE = 2ull << 62;
@@ -772,13 +773,13 @@ constexpr int LB_WIN = 1;            // look-back window: LB_WIN x 64 predecesso
 
 // K: the sort key type, (j << 6 | flop).  32-bit keys (columns < 2^26) halve
 // the key LDS and the sort's shuffles and registers.
-// With 32-bit keys the bound asks for five waves per SIMD (<= 102 VGPRs):
-// the look-back of one workgroup is hidden behind the other workgroups of
-// the CU (measured: 2% faster than four waves even with a few spilled
-// registers); 64-bit keys double the key LDS and ask for three.
+// The bound's second argument is the wavefronts per SIMD asked for; with
+// 32-bit keys SX_FUSED_WAVES, so that a workgroup waiting in its look-back
+// is covered by the others on the CU; 64-bit keys double the key LDS and ask
+// for three.
 // PK: 0 B read from Bp/Bj/Bx, 1 from the packed pairs, 2 from the row records
 template <class S, class K, int NT, int PK>
-__global__ __launch_bounds__(NT, (sizeof(K) == 4 ? 1280 : 768) / NT)
+__global__ __launch_bounds__(NT, sizeof(K) == 4 ? SX_FUSED_WAVES : 3)
 void k_sx_fused(SxArgs g, int64_t fs,
                                                          unsigned long long *status, int64_t ntiles,
                                                          int *err)
@@ -849,40 +850,60 @@ void k_sx_fused(SxArgs g, int64_t fs,
                 int32_t pres[TG];
                 #pragma unroll
                 for (int m = 0; m < TG; m++) pres[m] = __shfl(pre, m, TG);
-                #pragma unroll
-                for (int it = 0; it < FMAX / TG; it++)
+                // the chunk's flops in three passes, so that the B gathers of
+                // all FMAX/TG flops of a lane are in flight together: owners
+                // and B positions (a lane past the chunk's flops takes its
+                // last flop's, a valid position), the gathers, then the keys
+                if (tot > 0)
                 {
-                    const int64_t q = (int64_t) it * TG + sub;
-                    int lo = -1;
+                    constexpr int NIT = FMAX / TG;
+                    int los[NIT];
+                    int64_t pbs[NIT];
                     #pragma unroll
-                    for (int m = 0; m < TG; m++) lo += (pres[m] <= (int32_t) q) ? 1 : 0;
-                    lo = lo < 0 ? 0 : lo;
-                    // the owner's fields, shuffled by every lane of the team (a
-                    // lane outside the branch below would return zeros)
-                    const int64_t bsl = __shfl(bs, lo, TG);
-                    const int32_t prl = __shfl(pre, lo, TG);
-                    const Z avl = shfl_any<Z>(av, lo, TG);
-                    const int64_t kl = (S::mult == GM_MULT_K) ? __shfl(k, lo, TG) : 0;
-                    if (q < tot)
+                    for (int it = 0; it < NIT; it++)
                     {
-                        const int64_t pb = bsl + (q - prl);
-                        int64_t j;
-                        Z bv;
+                        const int64_t q = (int64_t) it * TG + sub;
+                        const int32_t qc = (int32_t) (q < tot ? q : tot - 1);
+                        int lo = -1;
+                        #pragma unroll
+                        for (int m = 0; m < TG; m++) lo += (pres[m] <= qc) ? 1 : 0;
+                        // shuffled by every lane of the team (a lane outside a
+                        // branch would return zeros)
+                        los[it] = lo;
+                        pbs[it] = __shfl(bs, lo, TG) + (qc - __shfl(pre, lo, TG));
+                    }
+                    int64_t js[NIT];
+                    uint64_t xb[NIT];
+                    #pragma unroll
+                    for (int it = 0; it < NIT; it++)
+                    {
                         if constexpr (PK != 0)
                         {
-                            const longlong2 jx = *(const longlong2 *) (g.Bjx + 2 * pb);
-                            j = (PK == 2) ? (jx.x & SX_REC_COL) : jx.x;
-                            bv = __builtin_bit_cast(Z, jx.y);
+                            const longlong2 jx = *(const longlong2 *) (g.Bjx + 2 * pbs[it]);
+                            js[it] = (PK == 2) ? (jx.x & SX_REC_COL) : jx.x;
+                            xb[it] = (uint64_t) jx.y;
                         }
                         else
-                            j = g.Bj[pb];
-                        if (sx_mask_ok(g, pm0, pm1, j))
+                        {
+                            js[it] = g.Bj[pbs[it]];
+                            xb[it] = 0;
+                        }
+                    }
+                    #pragma unroll
+                    for (int it = 0; it < NIT; it++)
+                    {
+                        const int64_t q = (int64_t) it * TG + sub;
+                        const Z avl = shfl_any<Z>(av, los[it], TG);
+                        const int64_t kl = (S::mult == GM_MULT_K) ? __shfl(k, los[it], TG) : 0;
+                        if (q < tot && sx_mask_ok(g, pm0, pm1, js[it]))
                         {
                             const int64_t fq = f + q;
-                            key[fq] = (K) (((K) j << 6) | (K) fq);
+                            key[fq] = (K) (((K) js[it] << 6) | (K) fq);
                             if constexpr (!S::iso_out)
                             {
-                                if constexpr (PK == 0) bv = S::geta((const Z *) g.Bx, pb, g.b_iso);
+                                Z bv;
+                                if constexpr (PK != 0) bv = __builtin_bit_cast(Z, xb[it]);
+                                else bv = S::geta((const Z *) g.Bx, pbs[it], g.b_iso);
                                 val[fq] = S::mul(avl, bv, kl);
                             }
                             nallow++;
