@@ -1132,19 +1132,16 @@ void k_sx_fused(SxArgs g, int64_t fs,
         const int64_t total = s_off[FUSED_ROWS];
         if (sub == 0 && i < g.m) g.Cp[i] = prefix + s_off[team];
         if (tile == ntiles - 1 && threadIdx.x == 0) g.Cp[g.m] = prefix + total;
-        // contiguous copy of the tile's small-row output
-        for (int64_t e = threadIdx.x; e < total; e += NT)
+        // each team copies its own small row: the row's entries are
+        // contiguous in C, its lanes store consecutive ones
+        if (!big && i < g.m)
         {
-            int lo = 0, hi = FUSED_ROWS - 1;
-            while (lo < hi)
+            const int64_t r0 = prefix + s_off[team];
+            for (int pos = sub; pos < (int) cnt; pos += TG)
             {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= e) lo = mid; else hi = mid - 1;
+                g.Ci[r0 + pos] = (int64_t) outj[pos];
+                if constexpr (!S::iso_out) ((Z *) g.Cx)[r0 + pos] = val[pos];
             }
-            if (s_big[lo]) continue;
-            const int pos = (int) (e - s_off[lo]);
-            g.Ci[prefix + e] = (int64_t) ((const int32_t *) s_key[lo])[pos];
-            if constexpr (!S::iso_out) ((Z *) g.Cx)[prefix + e] = s_val[lo][pos];
         }
         __syncthreads();
     }
