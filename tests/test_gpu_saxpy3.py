@@ -181,3 +181,52 @@ def test_saxpy3_fused_masked_bfs_shape(grb, mode):
     got = gm_saxpy3("GxB_ANY_SECONDI_INT64", F, G, M=V, mask_comp=True,
                     params=(64, 512, 2048, mode))
     assert_csr_equal(got, ref, what=f"bfs mode {mode}")
+
+
+def _short_rows_csr(n, ncols, maxlen, rng, vt, full=0.0):
+    """CSR with distinct sorted columns; a fraction `full` of the rows have
+    maxlen entries, the others a length uniform in [0, maxlen]."""
+    lens = np.where(rng.random(n) < full, maxlen, rng.integers(0, maxlen + 1, size=n))
+    p = np.zeros(n + 1, dtype=np.int64)
+    p[1:] = np.cumsum(lens)
+    j = np.concatenate([np.sort(rng.choice(ncols, size=int(l), replace=False)) for l in lens]
+                       ).astype(np.int64) if p[-1] else np.zeros(0, dtype=np.int64)
+    if vt == "fp64":
+        x = rng.random(int(p[-1]))
+    else:
+        x = rng.integers(-50, 50, size=int(p[-1])).astype(np.int64)
+    return W.Csr(n, ncols, p, j, x)
+
+
+@pytest.mark.parametrize("mask", ["none", "M", "!M"])
+@pytest.mark.parametrize("handle", ["GrB_PLUS_TIMES_SEMIRING_FP64", "GrB_PLUS_TIMES_SEMIRING_INT64",
+                                    "GrB_MIN_PLUS_SEMIRING_INT64"])
+def test_saxpy3_row_records_grb_mxm(grb, mask, handle):
+    """GrB_mxm with B's row records (rows of at most 16 entries at stride 16,
+    half of them full, so that the padding stays under the records' limit and
+    the fused kernel reads long rows in two groups of eight) and rows of A
+    whose flops cross the single-pass limit: against the oracle."""
+    print(f"[saxpy3] row records {handle} mask={mask}", flush=True)
+    rng = np.random.default_rng(77 + len(mask))
+    vt = "fp64" if "FP64" in handle else "int64"
+    n = 1500
+    A = _short_rows_csr(n, n, 9, rng, vt)
+    B = _short_rows_csr(n, n, 16, rng, vt, full=0.5)
+    pad = 16 * n - B.nnz
+    assert pad <= B.nnz // 2 + 1024, "B must qualify for row records"
+    Ad, Bd = capi.Matrix.from_csr(A, vt), capi.Matrix.from_csr(B, vt)
+    C = capi.Matrix(vt, n, n)
+    name, _ = SR[handle]
+    if mask == "none":
+        capi.mxm_checked(C, None, handle, Ad, Bd)
+        ref = O.saxpy3(name, A, B)
+    else:
+        M = rand_csr(n, n, 0.02, rng, "int64")
+        Md = capi.Matrix.from_csr(M, "int64")
+        d = capi.Descriptor(mask_struct=True, mask_comp=(mask == "!M"), axb=capi.GxB_AxB_SAXPY)
+        try:
+            capi.mxm_checked(C, Md, handle, Ad, Bd, d)
+        finally:
+            d.free()
+        ref = O.saxpy3(name, A, B, M=M, mask_comp=(mask == "!M"), mask_struct=True)
+    assert_csr_equal(C.to_csr(), ref, fp_tol=FP_TOL if vt == "fp64" else None, what=f"records {mask}")
