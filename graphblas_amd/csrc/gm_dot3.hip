@@ -1300,16 +1300,20 @@ __global__ __launch_bounds__(LC_NT) void k_dot3_lds(Dot3Args g, const uint64_t *
 // entries of a sorted bin that probe the same long row builds that row's
 // whole bitmap in LDS once (plus, for valued semirings, the row position of
 // the first entry of every 64-column word), and teams of G lanes probe their
-// short rows against it: lane l takes elements l, l+G, ... (four loads in
-// flight) and the partials combine in lane order, as in the global probe.
-// Other entries run the team kernel's own path.
+// short rows against it: lane l takes elements l, l+G, ... of the 32-bit
+// column copy (eight loads in flight) and the partials combine in lane
+// order, as in the global probe.  Other entries run the team kernel's own
+// path, without LDS staging: the kernel keeps its LDS small and its
+// registers capped at eight wavefronts per SIMD, because it is bound by the
+// latency of its entries' loads (4096-graph batch: 16-lane bin 387 -> 291
+// ms, wavefront bin 220 -> 164 ms).
 constexpr int L1_BITS = 1 << 16;
 constexpr int L1_WORDS = L1_BITS / 64;
 constexpr int L1_SLICE = 256;
 constexpr int L1_GMIN = 4;
 
 template <class S, int G, bool N32>
-__global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t *list,
+__global__ __launch_bounds__(D3_NT) __attribute__((amdgpu_waves_per_eu(8))) void k_dot3_lds1(Dot3Args g, const uint64_t *list,
                                                      const uint64_t *keys, int64_t count,
                                                      unsigned long long *ctr, const uint32_t *A32,
                                                      const uint32_t *B32)
@@ -1319,7 +1323,6 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
     constexpr bool RANK = !S::pattern;
     __shared__ uint64_t s_bits[L1_WORDS];
     __shared__ uint32_t s_rank[RANK ? L1_WORDS : 1];
-    __shared__ uint32_t s_lists[TPB][StageCap<G>::value];
     __shared__ uint64_t s_key[L1_SLICE];
     // every slice entry's short row (start, length) and probe direction
     // (0: none, 1: into A's row, 2: into B's row), set up together at the
@@ -1330,7 +1333,6 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
     __shared__ int64_t s_slice;
     __shared__ int s_ge;
     const int lane = threadIdx.x & (G - 1), team = threadIdx.x / G;
-    uint32_t *sl = s_lists[team];
     const Z *Ax = (const Z *) g.Ax;
     const Z *Bx = (const Z *) g.Bx;
     const int64_t nslices = (count + L1_SLICE - 1) / L1_SLICE;
@@ -1407,7 +1409,7 @@ __global__ __launch_bounds__(D3_NT) void k_dot3_lds1(Dot3Args g, const uint64_t 
                 {
                     if (!(use_lds && s_pdir[t] == (rowA ? 1 : 2)))
                     {
-                        team_task<S, G>(g, list[q0 + t], lane, sl);
+                        team_task<S, G>(g, list[q0 + t], lane, nullptr);
                         continue;
                     }
                     const int64_t p = (int64_t) (list[q0 + t] & 0xffffffffull);
