@@ -735,22 +735,57 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
 // serialize / deserialize (Include/GraphBLAS.h:10917-10990 of the reference)
 //------------------------------------------------------------------------------
 
-// Blob layout (this library's own wire format, little-endian, uncompressed):
-//   header  8 x uint64: magic "GMCSR001", type code, iso, nrows, ncols, nvals,
-//           value size, total blob size
-//   p       (nrows+1) int64 row pointers
-//   j       nvals int64 column indices (rows sorted)
-//   x       nvals values (1 if iso) of the value size
-// A blob from the reference's own format is rejected with
-// GrB_INVALID_OBJECT (its header does not carry this magic).
+// The blob is the reference's serialized matrix (its writer
+// Source/serialize/GB_serialize.c:300-390, header size and field macros
+// GB_serialize.h:106-140, reader GB_deserialize.c and
+// GB_deserialize_from_blob.c), little-endian:
+//   uint64 blob_size; int32 type code, version; int64 vlen, vdim, nvec,
+//   nvec_nonempty, nvals, type size, then the byte lengths of the arrays p, h,
+//   b, i, x; float hyper_switch, bitmap_switch; int32 sparsity_control,
+//   4*sparsity + 2*iso + is_csc; int32 (number of blocks, method) for each of
+//   p, h, b, i, x; for each array its blocks' cumulative end offsets (int64);
+//   the blocks of p, h, b, i, x in that order; the matrix name and the
+//   element type name, each ended by a nul byte.
+// This library writes every matrix sparse by row with each array as one
+// uncompressed block (method GxB_COMPRESSION_NONE, which the reference reads
+// as is; the descriptor's compression request is not honoured), and reads
+// sparse, hypersparse, bitmap and full blobs by row or by column whose blocks
+// are uncompressed or LZ4 / LZ4HC (decoded here); ZSTD blocks (the
+// reference's default) are reported as GrB_NOT_IMPLEMENTED.
 
-#define GB_BLOB_MAGIC 0x3130305253434d47ULL     // "GMCSR001" read as a little-endian uint64
-#define GB_BLOB_HDR 8
+#define GB_BLOB_HEADER (sizeof(uint64_t) + 11 * sizeof(int64_t) + 14 * sizeof(int32_t) + 2 * sizeof(float))
+#define GB_BLOB_VERSION ((int32_t) ((GxB_IMPLEMENTATION_MAJOR * 1000 + GxB_IMPLEMENTATION_MINOR) * 1000 + \
+                                    GxB_IMPLEMENTATION_SUB))
 
-static GrB_Index GB_blob_size(int64_t nrows, int64_t nvals, size_t xsize, bool iso)
+// the reference's element type names of the built-in types
+// (Source/get_set/GB_code_name_get.c)
+static const char *GB_blob_type_name(int32_t code)
 {
-    return (GrB_Index) (sizeof(uint64_t) * GB_BLOB_HDR + sizeof(int64_t) * (size_t) (nrows + 1) +
-                        sizeof(int64_t) * (size_t) nvals + xsize * (size_t) (iso ? 1 : nvals));
+    switch (code)
+    {
+        case GM_BOOL: return "GrB_BOOL";
+        case GM_INT8: return "GrB_INT8";
+        case GM_UINT8: return "GrB_UINT8";
+        case GM_INT16: return "GrB_INT16";
+        case GM_UINT16: return "GrB_UINT16";
+        case GM_INT32: return "GrB_INT32";
+        case GM_UINT32: return "GrB_UINT32";
+        case GM_INT64: return "GrB_INT64";
+        case GM_UINT64: return "GrB_UINT64";
+        case GM_FP32: return "GrB_FP32";
+        case GM_FP64: return "GrB_FP64";
+        default: return "";
+    }
+}
+
+static GrB_Index GB_blob_size(int64_t nrows, int64_t nvals, size_t xsize, bool iso, int32_t code)
+{
+    const size_t xn = xsize * (size_t) (iso ? 1 : nvals);
+    size_t s = GB_BLOB_HEADER;
+    s += sizeof(int64_t) * (1 + (nvals > 0 ? 1 : 0) + (xn > 0 ? 1 : 0));    // block ends
+    s += sizeof(int64_t) * (size_t) (nrows + 1) + sizeof(int64_t) * (size_t) nvals + xn;
+    s += 1 + strlen(GB_blob_type_name(code)) + 1;                           // names
+    return (GrB_Index) s;
 }
 
 GrB_Info GrB_Matrix_serializeSize(GrB_Index *blob_size_handle, GrB_Matrix A)
@@ -758,52 +793,89 @@ GrB_Info GrB_Matrix_serializeSize(GrB_Index *blob_size_handle, GrB_Matrix A)
     GrB_Info info = GB_check_matrix(A);
     if (info != GrB_SUCCESS) return info;
     if (!blob_size_handle) return GrB_NULL_POINTER;
-    *blob_size_handle = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    *blob_size_handle = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso, A->type->code);
     return GrB_SUCCESS;
 }
+
+#define GB_PUT(v, T)                              \
+    do {                                          \
+        const T v_ = (T) (v);                     \
+        memcpy(q, &v_, sizeof(T));                \
+        q += sizeof(T);                           \
+    } while (0)
 
 GrB_Info GrB_Matrix_serialize(void *blob, GrB_Index *blob_size_handle, GrB_Matrix A)
 {
     GrB_Info info = GB_check_matrix(A);
     if (info != GrB_SUCCESS) return info;
     if (!blob || !blob_size_handle) return GrB_NULL_POINTER;
-    const GrB_Index need = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    const int32_t code = A->type->code;
+    const GrB_Index need = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso, code);
     if (*blob_size_handle < need)
         return GB_error(A, GrB_INSUFFICIENT_SPACE, "serialize: blob of %llu bytes, %llu needed",
                         (unsigned long long) *blob_size_handle, (unsigned long long) need);
     int64_t *hp, *hj;
     void *hx;
     if ((info = GB_download(A, &hp, &hj, &hx)) != GrB_SUCCESS) return info;
-    uint64_t *h = (uint64_t *) blob;
-    h[0] = GB_BLOB_MAGIC;
-    h[1] = (uint64_t) A->type->code;
-    h[2] = A->iso ? 1 : 0;
-    h[3] = (uint64_t) A->nrows;
-    h[4] = (uint64_t) A->ncols;
-    h[5] = (uint64_t) A->nvals;
-    h[6] = (uint64_t) A->type->size;
-    h[7] = (uint64_t) need;
-    uint8_t *q = (uint8_t *) blob + sizeof(uint64_t) * GB_BLOB_HDR;
-    memcpy(q, hp, sizeof(int64_t) * (size_t) (A->nrows + 1));
-    q += sizeof(int64_t) * (size_t) (A->nrows + 1);
-    memcpy(q, hj, sizeof(int64_t) * (size_t) A->nvals);
-    q += sizeof(int64_t) * (size_t) A->nvals;
-    memcpy(q, hx, A->type->size * (size_t) (A->iso ? 1 : A->nvals));
+    const int64_t n = A->nrows, nz = A->nvals;
+    int64_t nonempty = 0;
+    for (int64_t i = 0; i < n; i++) nonempty += (hp[i + 1] > hp[i]) ? 1 : 0;
+    const int64_t p_len = (int64_t) sizeof(int64_t) * (n + 1);
+    const int64_t i_len = (int64_t) sizeof(int64_t) * nz;
+    const int64_t x_len = (int64_t) A->type->size * (A->iso ? 1 : nz);
+    const int32_t p_nb = 1, i_nb = i_len > 0 ? 1 : 0, x_nb = x_len > 0 ? 1 : 0;
+    uint8_t *q = (uint8_t *) blob;
+    GB_PUT(need, uint64_t);
+    GB_PUT(code, int32_t);
+    GB_PUT(GB_BLOB_VERSION, int32_t);
+    GB_PUT(A->ncols, int64_t);                  // vlen: the length of a row
+    GB_PUT(n, int64_t);                         // vdim: the number of rows
+    GB_PUT(n, int64_t);                         // nvec
+    GB_PUT(nonempty, int64_t);
+    GB_PUT(nz, int64_t);
+    GB_PUT(A->type->size, int64_t);
+    GB_PUT(p_len, int64_t);
+    GB_PUT(0, int64_t);                         // h
+    GB_PUT(0, int64_t);                         // b
+    GB_PUT(i_len, int64_t);
+    GB_PUT(x_len, int64_t);
+    GB_PUT(0.0625f, float);                     // hyper_switch (the reference's default)
+    GB_PUT(0.04f, float);                       // bitmap_switch
+    GB_PUT(A->sparsity_control, int32_t);
+    GB_PUT(4 * GxB_SPARSE + (A->iso ? 2 : 0), int32_t);    // by row
+    GB_PUT(p_nb, int32_t); GB_PUT(GxB_COMPRESSION_NONE, int32_t);
+    GB_PUT(0, int32_t);    GB_PUT(GxB_COMPRESSION_NONE, int32_t);
+    GB_PUT(0, int32_t);    GB_PUT(GxB_COMPRESSION_NONE, int32_t);
+    GB_PUT(i_nb, int32_t); GB_PUT(GxB_COMPRESSION_NONE, int32_t);
+    GB_PUT(x_nb, int32_t); GB_PUT(GxB_COMPRESSION_NONE, int32_t);
+    GB_PUT(p_len, int64_t);
+    if (i_nb) GB_PUT(i_len, int64_t);
+    if (x_nb) GB_PUT(x_len, int64_t);
+    memcpy(q, hp, (size_t) p_len);
+    q += p_len;
+    if (i_len > 0) memcpy(q, hj, (size_t) i_len);
+    q += i_len;
+    if (x_len > 0) memcpy(q, hx, (size_t) x_len);
+    q += x_len;
+    *q++ = 0;                                   // no matrix name
+    const char *tn = GB_blob_type_name(code);
+    memcpy(q, tn, strlen(tn) + 1);
+    q += strlen(tn) + 1;
     free(hp);
     free(hj);
     free(hx);
-    *blob_size_handle = need;
+    *blob_size_handle = (GrB_Index) (q - (uint8_t *) blob);
     return GrB_SUCCESS;
 }
 
 GrB_Info GxB_Matrix_serialize(void **blob_handle, GrB_Index *blob_size_handle, GrB_Matrix A,
                               const GrB_Descriptor desc)
 {
-    (void) desc;        // compression method / threads: this format is uncompressed
+    (void) desc;        // compression: every array is written as one uncompressed block
     GrB_Info info = GB_check_matrix(A);
     if (info != GrB_SUCCESS) return info;
     if (!blob_handle || !blob_size_handle) return GrB_NULL_POINTER;
-    GrB_Index size = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso);
+    GrB_Index size = GB_blob_size(A->nrows, A->nvals, A->type->size, A->iso, A->type->code);
     void *blob = malloc((size_t) size);
     if (!blob) return GrB_OUT_OF_MEMORY;
     info = GrB_Matrix_serialize(blob, &size, A);
@@ -817,54 +889,373 @@ GrB_Info GxB_Matrix_serialize(void **blob_handle, GrB_Index *blob_size_handle, G
     return GrB_SUCCESS;
 }
 
-// header and structure checks of a blob; no device access
-GrB_Info GB_blob_check(const void *blob, GrB_Index blob_size, GrB_Type *type_out)
+// One LZ4 block (the LZ4 block format: sequences of a token, literals and a
+// back-reference of 4 or more bytes; the last sequence has literals only)
+// into dst[0, cap); returns the decoded length, or -1 for a malformed block.
+static int64_t GB_lz4_decode(const uint8_t *src, size_t n, uint8_t *dst, size_t cap)
 {
-    if (!blob) return GrB_NULL_POINTER;
-    if (blob_size < sizeof(uint64_t) * GB_BLOB_HDR) return GrB_INVALID_OBJECT;
-    const uint64_t *h = (const uint64_t *) blob;
-    if (h[0] != GB_BLOB_MAGIC) return GrB_INVALID_OBJECT;
-    GrB_Type t = GB_type_from_code((int32_t) h[1]);
-    if (!t || h[6] != t->size || h[2] > 1) return GrB_INVALID_OBJECT;
-    const uint64_t nrows = h[3], nvals = h[5];
-    if (nrows >= ((uint64_t) 1 << 56) || nvals >= ((uint64_t) 1 << 56)) return GrB_INVALID_OBJECT;
-    if (h[7] != GB_blob_size((int64_t) nrows, (int64_t) nvals, t->size, h[2] != 0) ||
-        h[7] > blob_size)
-        return GrB_INVALID_OBJECT;
-    // row pointers monotone from 0 to nvals, columns in range and sorted per row
-    const int64_t *p = (const int64_t *) (h + GB_BLOB_HDR);
-    const int64_t *j = p + nrows + 1;
-    if (p[0] != 0 || (uint64_t) p[nrows] != nvals) return GrB_INVALID_OBJECT;
-    for (uint64_t i = 0; i < nrows; i++)
+    size_t ip = 0, op = 0;
+    while (ip < n)
     {
-        if (p[i + 1] < p[i]) return GrB_INVALID_OBJECT;
-        for (int64_t k = p[i]; k < p[i + 1]; k++)
+        const unsigned token = src[ip++];
+        size_t lit = token >> 4;
+        if (lit == 15)
         {
-            if (j[k] < 0 || (uint64_t) j[k] >= h[4]) return GrB_INVALID_OBJECT;
-            if (k > p[i] && j[k] <= j[k - 1]) return GrB_INVALID_OBJECT;
+            unsigned b;
+            do
+            {
+                if (ip >= n) return -1;
+                b = src[ip++];
+                lit += b;
+            } while (b == 255);
+        }
+        if (lit > n - ip || lit > cap - op) return -1;
+        memcpy(dst + op, src + ip, lit);
+        ip += lit;
+        op += lit;
+        if (ip == n) break;                     // the block ends with literals
+        if (n - ip < 2) return -1;
+        const size_t off = (size_t) src[ip] | ((size_t) src[ip + 1] << 8);
+        ip += 2;
+        if (off == 0 || off > op) return -1;
+        size_t ml = token & 15;
+        if (ml == 15)
+        {
+            unsigned b;
+            do
+            {
+                if (ip >= n) return -1;
+                b = src[ip++];
+                ml += b;
+            } while (b == 255);
+        }
+        ml += 4;
+        if (ml > cap - op) return -1;
+        for (size_t k = 0; k < ml; k++, op++) dst[op] = dst[op - off];   // may overlap
+    }
+    return (int64_t) op;
+}
+
+// Reads one array of len bytes from the blob at *s (its nb block ends in
+// ends[]), decoding its blocks; *out is malloc'd (NULL for len 0).
+static GrB_Info GB_blob_array(uint8_t **out, int64_t len, const uint8_t *blob, size_t size,
+                              const int64_t *ends, int32_t nb, int32_t method, size_t *s)
+{
+    *out = NULL;
+    if (len < 0 || nb < 0) return GrB_INVALID_OBJECT;
+    if (len == 0)
+    {
+        if (nb != 0 && !(nb == 1 && ends[0] == 0)) return GrB_INVALID_OBJECT;
+        return GrB_SUCCESS;
+    }
+    if (nb == 0 || ends[nb - 1] <= 0 || *s + (size_t) ends[nb - 1] > size ||
+        (method < 0 && ends[nb - 1] != len) || len / 255 > ends[nb - 1] + 16)
+        return GrB_INVALID_OBJECT;
+    uint8_t *x = (uint8_t *) malloc((size_t) len);
+    if (!x) return GrB_OUT_OF_MEMORY;
+    int64_t prev = 0;
+    for (int32_t k = 0; k < nb; k++)
+    {
+        // block k: bytes ends[k-1] .. ends[k] of the array's area in the blob,
+        // decoding to the k-th of nb equal parts of the array (GB_PARTITION)
+        const int64_t d0 = (k == 0) ? 0 : (int64_t) (((double) k * (double) len) / (double) nb);
+        const int64_t d1 = (k == nb - 1) ? len : (int64_t) (((double) (k + 1) * (double) len) / (double) nb);
+        const int64_t e = ends[k];
+        if (e <= prev || d1 <= d0 || *s + (size_t) e > size)
+        {
+            free(x);
+            return GrB_INVALID_OBJECT;
+        }
+        const uint8_t *src = blob + *s + prev;
+        const size_t ssz = (size_t) (e - prev), dsz = (size_t) (d1 - d0);
+        if (method < 0)
+        {
+            if (nb != 1 || (int64_t) ssz != len)
+            {
+                free(x);
+                return GrB_INVALID_OBJECT;
+            }
+            memcpy(x, src, ssz);
+        }
+        else if (method >= 1000 && method < 3000)
+        {
+            // an LZ4 block expands at most 255-fold
+            if (dsz > 255 * ssz + 16 || GB_lz4_decode(src, ssz, x + d0, dsz) != (int64_t) dsz)
+            {
+                free(x);
+                return GrB_INVALID_OBJECT;
+            }
+        }
+        else
+        {
+            free(x);
+            return GrB_NOT_IMPLEMENTED;         // ZSTD (method 0 and 3000 up)
+        }
+        prev = e;
+    }
+    *s += (size_t) prev;
+    *out = x;
+    return GrB_SUCCESS;
+}
+
+#define GB_GET(v, T)                                  \
+    T v;                                              \
+    memcpy(&v, blob + s, sizeof(T));                  \
+    s += sizeof(T)
+
+// A blob in host CSR form, by row: p (nrows+1), j, x (one value if iso)
+typedef struct
+{
+    GrB_Type type;
+    int64_t nrows, ncols, nvals;
+    bool iso;
+    int64_t *p, *j;
+    uint8_t *x;
+} GB_blob_csr;
+
+static void GB_blob_csr_free(GB_blob_csr *c)
+{
+    free(c->p);
+    free(c->j);
+    free(c->x);
+    c->p = c->j = NULL;
+    c->x = NULL;
+}
+
+// Parses and checks a blob into host CSR (no device access).
+static GrB_Info GB_blob_parse(const uint8_t *blob, size_t size, GB_blob_csr *c)
+{
+    memset(c, 0, sizeof(*c));
+    if (!blob) return GrB_NULL_POINTER;
+    if (size < GB_BLOB_HEADER) return GrB_INVALID_OBJECT;
+    size_t s = 0;
+    GB_GET(bsize, uint64_t);
+    GB_GET(code, int32_t);
+    GB_GET(version, int32_t);
+    GB_GET(vlen, int64_t);
+    GB_GET(vdim, int64_t);
+    GB_GET(nvec, int64_t);
+    GB_GET(nvec_nonempty, int64_t);
+    GB_GET(nvals, int64_t);
+    GB_GET(typesize, int64_t);
+    GB_GET(p_len, int64_t);
+    GB_GET(h_len, int64_t);
+    GB_GET(b_len, int64_t);
+    GB_GET(i_len, int64_t);
+    GB_GET(x_len, int64_t);
+    GB_GET(hyper_switch, float);
+    GB_GET(bitmap_switch, float);
+    GB_GET(sparsity_control, int32_t);
+    GB_GET(sic, int32_t);
+    int32_t nb[5], meth[5];
+    for (int a = 0; a < 5; a++)
+    {
+        memcpy(&nb[a], blob + s, sizeof(int32_t));
+        s += sizeof(int32_t);
+        memcpy(&meth[a], blob + s, sizeof(int32_t));
+        s += sizeof(int32_t);
+    }
+    (void) version; (void) nvec_nonempty; (void) hyper_switch; (void) bitmap_switch;
+    (void) sparsity_control;
+    if (bsize != (uint64_t) size || code < GM_BOOL || code > 14) return GrB_INVALID_OBJECT;
+    GrB_Type t = GB_type_from_code(code);
+    if (!t) return GrB_NOT_IMPLEMENTED;         // complex and user-defined types
+    if ((int64_t) t->size != typesize) return GrB_INVALID_OBJECT;
+    const int sparsity = sic / 4;
+    const bool iso = (sic & 2) != 0, csc = (sic & 1) != 0;
+    const int64_t lim = (int64_t) 1 << 60;
+    if (vlen < 0 || vdim < 0 || vlen >= lim || vdim >= lim || nvals < 0 || nvec < 0 || nvec > vdim)
+        return GrB_INVALID_OBJECT;
+    int64_t nball = 0;
+    for (int a = 0; a < 5; a++)
+    {
+        if (nb[a] < 0) return GrB_INVALID_OBJECT;
+        nball += nb[a];
+    }
+    if (s + sizeof(int64_t) * (size_t) nball > size) return GrB_INVALID_OBJECT;
+    int64_t *ends[5];
+    int64_t *endbuf = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nball > 0 ? nball : 1));
+    if (!endbuf) return GrB_OUT_OF_MEMORY;
+    memcpy(endbuf, blob + s, sizeof(int64_t) * (size_t) nball);
+    s += sizeof(int64_t) * (size_t) nball;
+    for (int a = 0, o = 0; a < 5; o += nb[a], a++) ends[a] = endbuf + o;
+    const int64_t lens[5] = {p_len, h_len, b_len, i_len, x_len};
+    uint8_t *arr[5] = {NULL, NULL, NULL, NULL, NULL};
+    GrB_Info info = GrB_SUCCESS;
+    for (int a = 0; a < 5 && info == GrB_SUCCESS; a++)
+        info = GB_blob_array(&arr[a], lens[a], blob, size, ends[a], nb[a], meth[a], &s);
+    free(endbuf);
+    // vectors (rows when by row, columns when by column) as pointers and
+    // indices: (vp[vdim+1], vi[nvals]) and the values' positions vx[] (or
+    // NULL for the stored order)
+    int64_t *vp = NULL, *vi = NULL, *vx = NULL;
+    const int64_t nx = iso ? 1 : nvals;
+    if (info == GrB_SUCCESS && (sparsity == GxB_SPARSE || sparsity == GxB_HYPERSPARSE))
+    {
+        const int64_t *P = (const int64_t *) arr[0], *H = (const int64_t *) arr[1];
+        const int64_t *I = (const int64_t *) arr[3];
+        const int64_t pv = (sparsity == GxB_HYPERSPARSE) ? nvec : vdim;
+        if (p_len != (int64_t) sizeof(int64_t) * (pv + 1) || i_len != (int64_t) sizeof(int64_t) * nvals ||
+            (sparsity == GxB_HYPERSPARSE && h_len != (int64_t) sizeof(int64_t) * nvec) ||
+            (sparsity == GxB_SPARSE && h_len != 0) || b_len != 0 ||
+            x_len != typesize * nx || P == NULL || P[0] != 0 || P[pv] != nvals)
+            info = GrB_INVALID_OBJECT;
+        vp = (int64_t *) calloc((size_t) vdim + 1, sizeof(int64_t));
+        vi = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1));
+        if (info == GrB_SUCCESS && (!vp || !vi)) info = GrB_OUT_OF_MEMORY;
+        for (int64_t k = 0; info == GrB_SUCCESS && k < pv; k++)
+        {
+            const int64_t v = (sparsity == GxB_HYPERSPARSE) ? H[k] : k;
+            if (P[k + 1] < P[k] || v < 0 || v >= vdim ||
+                (sparsity == GxB_HYPERSPARSE && k > 0 && v <= H[k - 1]))
+            {
+                info = GrB_INVALID_OBJECT;
+                break;
+            }
+            vp[v + 1] = P[k + 1] - P[k];
+            for (int64_t e = P[k]; e < P[k + 1]; e++)
+                if (I[e] < 0 || I[e] >= vlen || (e > P[k] && I[e] <= I[e - 1])) info = GrB_INVALID_OBJECT;
+        }
+        if (info == GrB_SUCCESS)
+        {
+            for (int64_t v = 0; v < vdim; v++) vp[v + 1] += vp[v];
+            if (nvals > 0) memcpy(vi, I, sizeof(int64_t) * (size_t) nvals);
         }
     }
-    if (type_out) *type_out = t;
-    return GrB_SUCCESS;
+    else if (info == GrB_SUCCESS && (sparsity == GxB_BITMAP || sparsity == GxB_FULL))
+    {
+        // every position of a bitmap that holds an entry, or of a full matrix
+        const double cells = (double) vlen * (double) vdim;
+        const int64_t nc = (int64_t) cells;
+        if (cells > (double) lim || p_len != 0 || h_len != 0 || i_len != 0 ||
+            (sparsity == GxB_BITMAP && b_len != nc) || (sparsity == GxB_FULL && b_len != 0) ||
+            x_len != typesize * (iso ? 1 : nc) || (sparsity == GxB_FULL && nvals != nc))
+            info = GrB_INVALID_OBJECT;
+        int64_t cnt = 0;
+        if (info == GrB_SUCCESS && sparsity == GxB_BITMAP)
+            for (int64_t q = 0; q < nc; q++) cnt += arr[2][q] ? 1 : 0;
+        if (info == GrB_SUCCESS && sparsity == GxB_FULL) cnt = nc;
+        if (info == GrB_SUCCESS && cnt != nvals) info = GrB_INVALID_OBJECT;
+        vp = (int64_t *) calloc((size_t) vdim + 1, sizeof(int64_t));
+        vi = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1));
+        vx = iso ? NULL : (int64_t *) malloc(sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1));
+        if (info == GrB_SUCCESS && (!vp || !vi || (!iso && !vx))) info = GrB_OUT_OF_MEMORY;
+        int64_t e = 0;
+        for (int64_t v = 0; info == GrB_SUCCESS && v < vdim; v++)
+        {
+            for (int64_t k = 0; k < vlen; k++)
+            {
+                const int64_t q = v * vlen + k;
+                if (sparsity == GxB_BITMAP && !arr[2][q]) continue;
+                vi[e] = k;
+                if (vx) vx[e] = q;
+                e++;
+            }
+            vp[v + 1] = e;
+        }
+    }
+    else if (info == GrB_SUCCESS)
+    {
+        info = GrB_INVALID_OBJECT;
+    }
+    // to CSR by row: the vectors are rows, or (by column) transpose them
+    if (info == GrB_SUCCESS)
+    {
+        const int64_t nrows = csc ? vlen : vdim, ncols = csc ? vdim : vlen;
+        int64_t *p = (int64_t *) calloc((size_t) nrows + 1, sizeof(int64_t));
+        int64_t *j = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nvals > 0 ? nvals : 1));
+        uint8_t *x = (uint8_t *) malloc((size_t) typesize * (size_t) (nx > 0 ? nx : 1));
+        if (!p || !j || !x)
+        {
+            free(p);
+            free(j);
+            free(x);
+            info = GrB_OUT_OF_MEMORY;
+        }
+        else
+        {
+            const uint8_t *X = arr[4];
+            if (!csc)
+            {
+                memcpy(p, vp, sizeof(int64_t) * (size_t) (nrows + 1));
+                if (nvals > 0) memcpy(j, vi, sizeof(int64_t) * (size_t) nvals);
+                for (int64_t e = 0; e < nx; e++)
+                    memcpy(x + (size_t) e * (size_t) typesize,
+                           X + (size_t) (vx ? vx[e] : e) * (size_t) typesize, (size_t) typesize);
+            }
+            else
+            {
+                // counting sort of the column vectors' entries by row; a row's
+                // entries come in ascending column order
+                for (int64_t e = 0; e < nvals; e++) p[vi[e] + 1]++;
+                for (int64_t r = 0; r < nrows; r++) p[r + 1] += p[r];
+                int64_t *next = (int64_t *) malloc(sizeof(int64_t) * (size_t) (nrows > 0 ? nrows : 1));
+                if (!next)
+                {
+                    info = GrB_OUT_OF_MEMORY;
+                }
+                else
+                {
+                    memcpy(next, p, sizeof(int64_t) * (size_t) nrows);
+                    for (int64_t col = 0; col < vdim; col++)
+                        for (int64_t e = vp[col]; e < vp[col + 1]; e++)
+                        {
+                            const int64_t d = next[vi[e]]++;
+                            j[d] = col;
+                            if (!iso)
+                                memcpy(x + (size_t) d * (size_t) typesize,
+                                       X + (size_t) (vx ? vx[e] : e) * (size_t) typesize, (size_t) typesize);
+                        }
+                    if (iso && nx > 0) memcpy(x, X, (size_t) typesize);
+                    free(next);
+                }
+            }
+            if (info == GrB_SUCCESS)
+            {
+                c->type = t;
+                c->nrows = nrows;
+                c->ncols = ncols;
+                c->nvals = nvals;
+                c->iso = iso;
+                c->p = p;
+                c->j = j;
+                c->x = x;
+            }
+            else
+            {
+                free(p);
+                free(j);
+                free(x);
+            }
+        }
+    }
+    free(vp);
+    free(vi);
+    free(vx);
+    for (int a = 0; a < 5; a++) free(arr[a]);
+    return info;
 }
 
 GrB_Info GrB_Matrix_deserialize(GrB_Matrix *C, GrB_Type type, const void *blob, GrB_Index blob_size)
 {
     if (!C) return GrB_NULL_POINTER;
     *C = NULL;
-    GrB_Type bt;
-    GrB_Info info = GB_blob_check(blob, blob_size, &bt);
+    GB_blob_csr c;
+    GrB_Info info = GB_blob_parse((const uint8_t *) blob, (size_t) blob_size, &c);
     if (info != GrB_SUCCESS) return info;
-    if (type && type != bt) return GrB_DOMAIN_MISMATCH;
-    const uint64_t *h = (const uint64_t *) blob;
-    const int64_t nrows = (int64_t) h[3], ncols = (int64_t) h[4], nvals = (int64_t) h[5];
+    if (type && type != c.type)
+    {
+        GB_blob_csr_free(&c);
+        return GrB_DOMAIN_MISMATCH;
+    }
     GrB_Matrix A;
-    if ((info = GrB_Matrix_new(&A, bt, (GrB_Index) nrows, (GrB_Index) ncols)) != GrB_SUCCESS)
+    if ((info = GrB_Matrix_new(&A, c.type, (GrB_Index) c.nrows, (GrB_Index) c.ncols)) != GrB_SUCCESS)
+    {
+        GB_blob_csr_free(&c);
         return info;
-    const int64_t *p = (const int64_t *) (h + GB_BLOB_HDR);
-    const int64_t *j = p + nrows + 1;
-    const void *x = (const void *) (j + nvals);
-    info = GB_upload(A, p, j, x, nvals, h[2] != 0);
+    }
+    info = GB_upload(A, c.p, c.j, c.x, c.nvals, c.iso);
+    GB_blob_csr_free(&c);
     if (info != GrB_SUCCESS)
     {
         GrB_Matrix_free(&A);

@@ -213,8 +213,10 @@ GrB_Info GxB_Matrix_unpack_CSR(GrB_Matrix A, GrB_Index **Ap, GrB_Index **Aj, voi
                                bool *iso, bool *jumbled, const GrB_Descriptor desc);
 
 // Serialize to / from a blob (Include/GraphBLAS.h:10917-10990 of the
-// reference).  The blob is this library's own uncompressed CSR format; a blob
-// in the reference's format is rejected with GrB_INVALID_OBJECT.
+// reference), in the reference's blob layout (Source/serialize/GB_serialize.c).
+// Written: sparse by row, every array one uncompressed block.  Read: sparse,
+// hypersparse, bitmap or full, by row or by column, uncompressed or LZ4 /
+// LZ4HC blocks; ZSTD blocks give GrB_NOT_IMPLEMENTED.
 GrB_Info GxB_Matrix_serialize(void **blob_handle, GrB_Index *blob_size_handle, GrB_Matrix A,
                               const GrB_Descriptor desc);
 GrB_Info GrB_Matrix_serialize(void *blob, GrB_Index *blob_size_handle, GrB_Matrix A);
@@ -326,6 +328,14 @@ typedef enum
 } GxB_Option_Field;
 
 typedef enum { GxB_BY_ROW = 0, GxB_BY_COL = 1, GxB_NO_FORMAT = -1 } GxB_Format_Value;
+
+// serialize compression methods (GrB_Matrix_serialize writes uncompressed
+// blocks; deserialize reads uncompressed and LZ4 / LZ4HC blocks)
+#define GxB_COMPRESSION_NONE -1
+#define GxB_COMPRESSION_DEFAULT 0
+#define GxB_COMPRESSION_LZ4 1000
+#define GxB_COMPRESSION_LZ4HC 2000
+#define GxB_COMPRESSION_ZSTD 3000
 
 #define GxB_HYPERSPARSE 1   // treated as sparse
 #define GxB_SPARSE      2

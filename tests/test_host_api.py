@@ -269,30 +269,48 @@ def test_vector_objects_without_gpu():
 
 
 def test_deserialize_rejects_bad_blobs():
-    """Header and structure checks of GrB_Matrix_deserialize run on the host."""
+    """Header and structure checks of GrB_Matrix_deserialize run on the host
+    (blobs in the reference's layout, built by tests/blobs.py)."""
+    import blobs as BL
     capi.init()
     L = capi.lib()
     out = ctypes.c_void_p()
-    hdr = np.zeros(8, dtype=np.uint64)
-    # wrong magic
-    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, hdr.ctypes.data, hdr.nbytes) == \
+
+    def des(b, t=None):
+        buf = np.frombuffer(b, dtype=np.uint8).copy()
+        return L.GrB_Matrix_deserialize(ctypes.byref(out), capi.handle(t) if t else None,
+                                        buf.ctypes.data, buf.nbytes)
+
+    # a 2 x 3 int64 matrix with one entry, but its column out of range
+    arrs = BL.csr_arrays([0, 1, 1], [5], [7])
+    assert des(BL.blob(3, 2, 8, 8, arrs, nvals=1)) == capi.GrB_INVALID_OBJECT
+    good = BL.blob(3, 2, 8, 8, BL.csr_arrays([0, 1, 1], [2], [7]), nvals=1)
+    # a header shorter than the layout's, a truncated blob, a size field that
+    # disagrees with the blob
+    assert des(good[:100]) == capi.GrB_INVALID_OBJECT
+    assert des(good[:-8]) == capi.GrB_INVALID_OBJECT
+    assert des(good + b"\0") == capi.GrB_INVALID_OBJECT
+    # a type code outside the reference's, and a type size that disagrees
+    assert des(BL.blob(3, 2, 99, 8, BL.csr_arrays([0, 1, 1], [2], [7]), nvals=1)) == \
         capi.GrB_INVALID_OBJECT
-    # right magic, 2x3 int64 matrix with 1 entry, but a column out of range
-    magic = int.from_bytes(b"GMCSR001", "little")
-    p = np.array([0, 1, 1], dtype=np.int64)
-    j = np.array([5], dtype=np.int64)
-    x = np.array([7], dtype=np.int64)
-    size = 8 * 8 + p.nbytes + j.nbytes + x.nbytes
-    blob = np.concatenate([np.array([magic, 8, 0, 2, 3, 1, 8, size], dtype=np.uint64).view(np.uint8),
-                           p.view(np.uint8), j.view(np.uint8), x.view(np.uint8)])
-    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, blob.ctypes.data, blob.nbytes) == \
-        capi.GrB_INVALID_OBJECT
-    # truncated blob
-    assert L.GrB_Matrix_deserialize(ctypes.byref(out), None, blob.ctypes.data, blob.nbytes - 8) == \
+    assert des(BL.blob(3, 2, 8, 4, BL.csr_arrays([0, 1, 1], [2], [7]), nvals=1)) == \
         capi.GrB_INVALID_OBJECT
     # a type that differs from the blob's
-    j[0] = 2
-    blob = np.concatenate([np.array([magic, 8, 0, 2, 3, 1, 8, size], dtype=np.uint64).view(np.uint8),
-                           p.view(np.uint8), j.view(np.uint8), x.view(np.uint8)])
-    assert L.GrB_Matrix_deserialize(ctypes.byref(out), capi.handle("GrB_FP64"), blob.ctypes.data,
-                                    blob.nbytes) == capi.GrB_DOMAIN_MISMATCH
+    assert des(good, "GrB_FP64") == capi.GrB_DOMAIN_MISMATCH
+    # row pointers that do not end at nvals, columns out of order
+    assert des(BL.blob(3, 2, 8, 8, BL.csr_arrays([0, 1, 2], [2], [7]), nvals=1)) == \
+        capi.GrB_INVALID_OBJECT
+    assert des(BL.blob(3, 2, 8, 8, BL.csr_arrays([0, 2, 2], [2, 1], [7, 8]), nvals=2)) == \
+        capi.GrB_INVALID_OBJECT
+    # ZSTD blocks (the reference's default compression) are not decoded
+    arrs = BL.csr_arrays([0, 1, 1], [2], [7])
+    zs = BL.blob(3, 2, 8, 8, arrs, nvals=1, methods={"p": BL.ZSTD},
+                 blocks={"p": [b"\x28\xb5\x2f\xfd" + bytes(20)]})
+    assert des(zs) == capi.GrB_NOT_IMPLEMENTED
+    # an LZ4 block that decodes to fewer bytes than the array's length
+    bad = BL.blob(3, 2, 8, 8, arrs, nvals=1, methods={"i": BL.LZ4},
+                  blocks={"i": [BL.lz4_literals(arrs["i"][:4])]})
+    assert des(bad) == capi.GrB_INVALID_OBJECT
+    # a back-reference before the start of the output
+    assert des(BL.blob(3, 2, 8, 8, arrs, nvals=1, methods={"i": BL.LZ4},
+                       blocks={"i": [bytes([0x14, 1, 9, 0])]})) == capi.GrB_INVALID_OBJECT
