@@ -109,7 +109,8 @@ class GmDot3Params(ctypes.Structure):
 
 class GmSaxpy3Params(ctypes.Structure):
     _fields_ = [("wave_small_max", ctypes.c_int64), ("wave_large_max", ctypes.c_int64),
-                ("block_max", ctypes.c_int64), ("mode", ctypes.c_int64)]
+                ("block_max", ctypes.c_int64), ("mode", ctypes.c_int64),
+                ("table", ctypes.c_int64)]
 
 
 class GmDot3Request(ctypes.Structure):

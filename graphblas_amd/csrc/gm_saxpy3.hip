@@ -73,6 +73,14 @@ struct SxArgs
     // its length in the top byte of the first record's column)
     const int64_t *Bjx;
     int32_t rec_lg;
+    // row tables: 0 the reference's rule (a table that holds every column of
+    // C is indexed by the column itself, Gustavson's dense workspace; a
+    // global-bin row whose hash table would be at least ncols/12 slots, or
+    // whose flops reach ncols/2, gets such a table), 1 always hash
+    // (GxB_AxB_HASH), 2 dense wherever a table holds every column or a
+    // global-bin row's dense table is at most 16 times its hash table
+    // (GxB_AxB_GUSTAVSON)
+    int32_t table;
 };
 
 __device__ __forceinline__ void wave_sync()
@@ -126,11 +134,13 @@ __device__ __forceinline__ bool sx_mask_ok(const SxArgs &g, int64_t pm0, int64_t
     return (g.mode == MODE_MASK) ? in : !in;
 }
 
-// insert key; returns its slot and whether this call created it
-__device__ __forceinline__ int32_t sx_insert(uint32_t *keys, int logT, uint32_t key, bool &created)
+// insert key; returns its slot and whether this call created it.  dense: the
+// table has a slot for every column, the key's own (no probing happens)
+__device__ __forceinline__ int32_t sx_insert(uint32_t *keys, int logT, uint32_t key, bool &created,
+                                             bool dense)
 {
     const uint32_t mask = (1u << logT) - 1u;
-    uint32_t h = sx_hash(key, logT);
+    uint32_t h = dense ? key : sx_hash(key, logT);
     created = false;
     while (true)
     {
@@ -296,8 +306,13 @@ __global__ __launch_bounds__(SX_NT) void k_sx_flops(SxArgs g, int64_t fs, int li
             g.rows[(int64_t) bin * g.m + (int64_t) slot] = (uint32_t) i;
             if (bin == BIN_GLB)
             {
-                // table of 2^ceil(log2(2t)) slots, load factor <= 1/2
-                const int lg = ilog2_ceil(2 * t);
+                // table of 2^ceil(log2(2t)) slots, load factor <= 1/2, or a
+                // dense one of 2^ceil(log2(ncols)) (see SxArgs::table)
+                int lg = ilog2_ceil(2 * t);
+                const int lgd = ilog2_ceil(g.n);
+                const bool dn = (g.table == 0 && (((int64_t) 1 << lg) >= g.n / 12 || t >= g.n / 2)) ||
+                                (g.table == 2 && lgd <= lg + 4);
+                if (dn && lgd > lg) lg = lgd;
                 const unsigned long long sz = 1ull << lg;
                 g.gtab_off[slot] = (int64_t) atomicAdd(&g.cnt[SX_NBINS], sz);
                 g.gtab_log2[slot] = lg;
@@ -442,6 +457,7 @@ __device__ void sx_row(const SxArgs &g, int64_t i, uint32_t *keys, typename S::Z
     const int lane = threadIdx.x & 63;
     const int wv = (NT == 64) ? 0 : (threadIdx.x >> 6);
     const int64_t T = (int64_t) 1 << logT;
+    const bool dense = g.table != 1 && T >= g.n;    // Gustavson: index by column
 
     for (int64_t t = tid; t < T; t += NT)
     {
@@ -505,7 +521,7 @@ __device__ void sx_row(const SxArgs &g, int64_t i, uint32_t *keys, typename S::Z
                 {
                     if (PHASE)
                         tv = S::mul(s_av[lo], S::geta((const Z *) g.Bx, pb, g.b_iso), s_k[lo]);
-                    slot = sx_insert(keys, logT, (uint32_t) j, created);
+                    slot = sx_insert(keys, logT, (uint32_t) j, created, dense);
                 }
             }
             if (PHASE == 0)
@@ -1238,6 +1254,8 @@ static int saxpy3_run(gm_csr_out *C, const gm_csr *M, bool mask_comp, bool mask_
     g.gkeys = nullptr; g.gvals = nullptr; g.glist = nullptr;
     g.Bjx = nullptr;
     g.rec_lg = -1;
+    g.table = prm ? (int32_t) prm->table : 0;
+    if (g.table < 0 || g.table > 2) return set_error(GM_INVALID_VALUE, "saxpy3: table %d", (int) g.table);
     const gm_saxpy3_params tu = saxpy3_tuned();
     g.ws_max = pick_param(prm ? prm->wave_small_max : 0, tu.wave_small_max, 64);
     g.wl_max = pick_param(prm ? prm->wave_large_max : 0, tu.wave_large_max, 512);

@@ -354,9 +354,15 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
     }
     else
     {
+        // the descriptor's Hash / Gustavson hint picks the rows' tables
+        // (Include/GraphBLAS.h:532-534 of the reference); otherwise its rule
+        gm_saxpy3_params sp;
+        memset(&sp, 0, sizeof(sp));
+        sp.table = (axb == GxB_AxB_HASH) ? 1 : (axb == GxB_AxB_GUSTAVSON) ? 2 : 0;
         if (GB_burble)
-            printf("burble: GrB_mxm C%s=A*B %s: saxpy3 (hash), A nnz %lld B nnz %lld\n",
+            printf("burble: GrB_mxm C%s=A*B %s: saxpy3 (%s), A nnz %lld B nnz %lld\n",
                    Mask ? (mask_comp ? "<!M>" : "<M>") : "", GB_sr_name(&sr, nm, sizeof(nm)),
+                   sp.table == 1 ? "hash" : sp.table == 2 ? "gustavson" : "hash/gustavson",
                    (long long) (a_tran ? AT.nvals : A->nvals),
                    (long long) (b_tran ? BT.nvals : B->nvals));
         // B's row records (built once per content of B) when the fused kernel
@@ -364,7 +370,7 @@ GrB_Info GrB_mxm(GrB_Matrix C, const GrB_Matrix Mask, const GrB_BinaryOp accum,
         if (!b_tran && !B->iso && B->x && B->type->size == 8 && sr.mult_op != GM_MULT_PAIR &&
             sr.mult_op != GM_MULT_K && 2 * (a_tran ? AT.nvals : A->nvals) >= B->nrows)
             GB_matrix_aux(&vb, B);
-        rc = gm_saxpy3(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, NULL);
+        rc = gm_saxpy3(&R, Mask ? &vm : NULL, mask_comp, mstruct, &va, &vb, &sr, &sp);
     }
     GB_free_out(&AT);
     GB_free_out(&BT);

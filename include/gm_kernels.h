@@ -202,6 +202,10 @@ typedef struct
                               // larger: workgroup per row, global hash table
     int64_t mode;             // 0 auto, 1 prefer the single-pass fused kernel
                               // for small rows, 2 symbolic + numeric only
+    int64_t table;            // rows with a table: 0 the reference's Hash /
+                              // Gustavson rule, 1 hash (GxB_AxB_HASH), 2 dense
+                              // Gustavson workspace where it fits
+                              // (GxB_AxB_GUSTAVSON); not a tuning default
 } gm_saxpy3_params;
 
 //------------------------------------------------------------------------------

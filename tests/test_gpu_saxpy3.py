@@ -230,3 +230,48 @@ def test_saxpy3_row_records_grb_mxm(grb, mask, handle):
             d.free()
         ref = O.saxpy3(name, A, B, M=M, mask_comp=(mask == "!M"), mask_struct=True)
     assert_csr_equal(C.to_csr(), ref, fp_tol=FP_TOL if vt == "fp64" else None, what=f"records {mask}")
+
+
+@pytest.mark.parametrize("table", [0, 1, 2])
+@pytest.mark.parametrize("ncols,bins", [(300, (8, 16, 64, 2)), (3000, (8, 16, 64, 2)),
+                                        (3000, (8, 512, 2048, 2)), (40000, (8, 16, 32, 2))])
+@pytest.mark.parametrize("handle", ["GxB_PLUS_PAIR_INT64", "GrB_PLUS_TIMES_SEMIRING_FP64",
+                                    "GrB_MIN_PLUS_SEMIRING_INT32"])
+def test_saxpy3_hash_and_gustavson_tables(grb, table, ncols, bins, handle):
+    """Row tables indexed by hashing (table 1), by the column itself where a
+    table holds every column of C (table 2, Gustavson's dense workspace, also
+    for global-bin rows), or by the reference's rule (0); with the bins forced
+    small so that rows land in the LDS tables of 128 / 1024 / 4096 slots and
+    in HBM tables, and with a mask; all against the oracle."""
+    print(f"[saxpy3] table={table} ncols={ncols} bins={bins} {handle}", flush=True)
+    rng = np.random.default_rng(ncols + table)
+    vt = _vals(handle)
+    A = power_law_csr(400, 10, rng, values=vt)
+    B = rand_csr(400, ncols, min(0.5, 40.0 / ncols), rng, vt)
+    M = rand_csr(400, ncols, min(0.5, 60.0 / ncols), rng, "int64")
+    name, _ = SR[handle]
+    tol = FP_TOL if "fp64" in name else None
+    params = tuple(bins) + (table,)
+    ref = O.saxpy3(name, A, B)
+    got = gm_saxpy3(handle, A, B, params=params)
+    assert_csr_equal(got, ref, fp_tol=tol, what=f"table {table}")
+    ref = O.saxpy3(name, A, B, M=M, mask_comp=True, mask_struct=True)
+    got = gm_saxpy3(handle, A, B, M=M, mask_comp=True, params=params)
+    assert_csr_equal(got, ref, fp_tol=tol, what=f"table {table} <!M>")
+
+
+@pytest.mark.parametrize("axb", ["hash", "gustavson"])
+def test_saxpy3_method_hints_grb_mxm(grb, axb):
+    """GrB_mxm with the GxB_AxB_HASH / GxB_AxB_GUSTAVSON descriptor hints."""
+    print(f"[saxpy3] GrB_mxm hint {axb}", flush=True)
+    rng = np.random.default_rng(3)
+    A = power_law_csr(500, 12, rng, values="fp64")
+    B = rand_csr(500, 2000, 0.02, rng, "fp64")
+    Ad, Bd = capi.Matrix.from_csr(A, "fp64"), capi.Matrix.from_csr(B, "fp64")
+    C = capi.Matrix("fp64", 500, 2000)
+    d = capi.Descriptor(axb=capi.GxB_AxB_HASH if axb == "hash" else capi.GxB_AxB_GUSTAVSON)
+    try:
+        capi.mxm_checked(C, None, "GrB_PLUS_TIMES_SEMIRING_FP64", Ad, Bd, d)
+    finally:
+        d.free()
+    assert_csr_equal(C.to_csr(), O.saxpy3("plus_times_fp64", A, B), fp_tol=FP_TOL, what=axb)
