@@ -228,6 +228,7 @@ def lib():
         "gm_arena_bytes_in_use": ([], I64),
         "gm_arena_bytes_cached": ([], I64),
         "gm_debug_fail_after": ([I64], None),
+        "gm_debug_dot3_slice_max": ([I64], None),
         "gm_memcpy_h2d": ([P, P, ctypes.c_size_t], ctypes.c_int),
         "gm_memcpy_d2h": ([P, P, ctypes.c_size_t], ctypes.c_int),
         "gm_memcpy_d2d": ([P, P, ctypes.c_size_t], ctypes.c_int),

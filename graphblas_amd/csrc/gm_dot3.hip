@@ -29,6 +29,7 @@
 
 #include <cstring>
 #include <type_traits>
+#include <algorithm>
 #include <rocprim/device/device_radix_sort.hpp>
 #include "gm_internal.h"
 #include "gm_block.h"
@@ -2815,7 +2816,6 @@ extern "C" bool gm_dot3_branch(const gm_csr *M, bool mask_struct, const gm_csr *
     if (!operand_ok(sr, A) || !operand_ok(sr, B)) return false;
     // C<M> = A*B': M is nrows(A) x nrows(B), A and B share their column space
     if (M->nrows != A->nrows || M->ncols != B->nrows || A->ncols != B->ncols) return false;
-    if (M->nrows >= ((int64_t) 1 << 32)) return false;
     return true;
 }
 
@@ -2842,13 +2842,9 @@ extern "C" int gm_AxB_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, con
     return gm_dot3(C, M, mask_struct, A, B, sr, nullptr);
 }
 
-extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
-                       const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
+static int dot3_dispatch(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                         const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
 {
-    if (!C || !M || !A || !B || !sr) return set_error(GM_INVALID_VALUE, "gm_dot3: NULL argument");
-    if (!gm_dot3_branch(M, mask_struct, A, B, sr))
-        return set_error(GM_NO_VALUE, "gm_dot3: case not handled by the GPU kernels");
-    GM_TRY(ensure_init());
     int rc = GM_NO_VALUE;
 #define GM_DOT3_CASE(ADD, MULT, ZC, Z)                                         \
     if (sr->add_op == ADD && sr->mult_op == MULT && sr->ztype == ZC)           \
@@ -2860,6 +2856,160 @@ extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const g
     GM_SEMIRING_LIST(GM_DOT3_CASE)
 #undef GM_DOT3_CASE
     return rc;
+}
+
+// The deferred entries are packed as (row << 32 | mask position), so one run
+// takes fewer than 2^32 mask entries and rows.  A larger mask is cut into row
+// slices below that (g_dot3_slice_max entries; tests lower it), each run on
+// its own (its mask row pointers re-based to 0, A's rows viewed in place), and
+// the slices' results are joined.  No counterpart in the reference, whose
+// CPU kernels index with 64-bit integers throughout.
+static int64_t g_dot3_slice_max = (int64_t) 1 << 32;
+
+extern "C" void gm_debug_dot3_slice_max(int64_t v)
+{
+    g_dot3_slice_max = (v > 0 && v < ((int64_t) 1 << 32)) ? v : ((int64_t) 1 << 32);
+}
+
+__global__ void k_d3_rebase(const int64_t *__restrict__ src, int64_t n, int64_t base, int64_t add,
+                            int64_t *__restrict__ dst)
+{
+    for (int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t) gridDim.x * blockDim.x)
+        dst[t] = src[t] - base + add;
+}
+
+static int dot3_sliced(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                       const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params,
+                       int64_t slice_max)
+{
+    hipStream_t st = stream();
+    const int64_t n = M->nrows;
+    std::vector<int64_t> hp((size_t) n + 1);
+    GM_HIP(hipMemcpyAsync(hp.data(), M->p, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st),
+           "dot3 slices: Mp");
+    GM_HIP(hipStreamSynchronize(st), "dot3 slices: Mp sync");
+    const int64_t row_max = ((int64_t) 1 << 31);
+    // row slices: as many rows as fit below slice_max entries (at least one)
+    std::vector<int64_t> cut{0};
+    while (cut.back() < n)
+    {
+        const int64_t r0 = cut.back();
+        int64_t lo = r0 + 1, hi = (n < r0 + row_max) ? n : r0 + row_max;
+        while (lo < hi)
+        {
+            const int64_t mid = lo + (hi - lo + 1) / 2;
+            if (hp[(size_t) mid] - hp[(size_t) r0] < slice_max) lo = mid; else hi = mid - 1;
+        }
+        if (hp[(size_t) lo] - hp[(size_t) r0] >= ((int64_t) 1 << 32))
+            return set_error(GM_NO_VALUE, "dot3: a mask row of 2^32 entries or more");
+        cut.push_back(lo);
+    }
+    const size_t ns = cut.size() - 1;
+    std::vector<gm_csr_out> part(ns);
+    for (auto &o : part) memset(&o, 0, sizeof(o));
+    auto free_parts = [&]() {
+        for (auto &o : part)
+        {
+            gm_free(o.p);
+            gm_free(o.j);
+            gm_free(o.x);
+        }
+    };
+    DevBuf mp;
+    int64_t rmax = 0;
+    for (size_t k = 0; k < ns; k++) rmax = std::max(rmax, cut[k + 1] - cut[k]);
+    if (!mp.alloc(sizeof(int64_t) * (size_t) (rmax + 1))) return set_error(GM_OUT_OF_MEMORY, "dot3 slices");
+    const size_t msz = type_size(M->type);
+    for (size_t k = 0; k < ns; k++)
+    {
+        const int64_t r0 = cut[k], r1 = cut[k + 1], base = hp[(size_t) r0];
+        k_d3_rebase<<<grid_for(r1 - r0 + 1, 256, (int64_t) cu_count() * 8), 256, 0, st>>>(
+            M->p + r0, r1 - r0 + 1, base, 0, mp.as<int64_t>());
+        GM_HIP(hipGetLastError(), "k_d3_rebase");
+        gm_csr Ms = *M, As = *A;
+        Ms.nrows = r1 - r0;
+        Ms.p = mp.as<int64_t>();
+        Ms.j = M->j + base;
+        if (M->x && !M->iso) Ms.x = (const char *) M->x + (size_t) base * msz;
+        Ms.aux = nullptr;
+        As.nrows = r1 - r0;
+        As.p = A->p + r0;               // positions stay absolute in A's arrays
+        As.aux = nullptr;
+        const int rc = dot3_dispatch(&part[k], &Ms, mask_struct, &As, B, sr, params);
+        if (rc != GM_SUCCESS)
+        {
+            free_parts();
+            return rc;
+        }
+    }
+    // join: row pointers shifted by the entries of the slices before
+    int64_t total = 0;
+    for (auto &o : part) total += o.nvals;
+    const bool iso = part[0].iso != 0;
+    const size_t zs = type_size(part[0].type);
+    gm_csr_out R;
+    memset(&R, 0, sizeof(R));
+    R.nrows = n;
+    R.ncols = M->ncols;
+    R.nvals = total;
+    R.iso = iso ? 1 : 0;
+    R.type = part[0].type;
+    R.p = (int64_t *) gm_malloc(sizeof(int64_t) * (size_t) (n + 1));
+    R.j = (int64_t *) gm_malloc(sizeof(int64_t) * (size_t) (total > 0 ? total : 1));
+    R.x = gm_malloc(zs * (size_t) (iso ? 1 : (total > 0 ? total : 1)));
+    if (!R.p || !R.j || !R.x)
+    {
+        gm_free(R.p);
+        gm_free(R.j);
+        gm_free(R.x);
+        free_parts();
+        return set_error(GM_OUT_OF_MEMORY, "dot3 slices: result");
+    }
+    int64_t off = 0;
+    for (size_t k = 0; k < ns; k++)
+    {
+        const int64_t r0 = cut[k], r1 = cut[k + 1];
+        const gm_csr_out &o = part[k];
+        k_d3_rebase<<<grid_for(r1 - r0 + 1, 256, (int64_t) cu_count() * 8), 256, 0, st>>>(
+            o.p, r1 - r0 + 1, 0, off, R.p + r0);
+        GM_HIP(hipGetLastError(), "k_d3_rebase");
+        if (o.nvals > 0)
+        {
+            GM_HIP(hipMemcpyAsync(R.j + off, o.j, sizeof(int64_t) * (size_t) o.nvals, hipMemcpyDeviceToDevice, st),
+                   "dot3 slices: join j");
+            if (!iso)
+                GM_HIP(hipMemcpyAsync((char *) R.x + (size_t) off * zs, o.x, zs * (size_t) o.nvals,
+                                      hipMemcpyDeviceToDevice, st), "dot3 slices: join x");
+        }
+        off += o.nvals;
+    }
+    if (iso) GM_HIP(hipMemcpyAsync(R.x, part[0].x, zs, hipMemcpyDeviceToDevice, st), "dot3 slices: iso");
+    GM_HIP(hipStreamSynchronize(st), "dot3 slices: join sync");
+    free_parts();
+    *C = R;
+    return GM_SUCCESS;
+}
+
+extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
+                       const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
+{
+    if (!C || !M || !A || !B || !sr) return set_error(GM_INVALID_VALUE, "gm_dot3: NULL argument");
+    if (!gm_dot3_branch(M, mask_struct, A, B, sr))
+        return set_error(GM_NO_VALUE, "gm_dot3: case not handled by the GPU kernels");
+    GM_TRY(ensure_init());
+    if (M->nrows > 0 && M->p)
+    {
+        int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
+        if (!hp) return set_error(GM_OUT_OF_MEMORY, "pinned scratch");
+        hipStream_t st = stream();
+        GM_HIP(hipMemcpyAsync(hp, M->p + M->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(M)");
+        GM_HIP(hipStreamSynchronize(st), "nnz(M) sync");
+        const int64_t slice_max = g_dot3_slice_max;
+        if (*hp >= slice_max || M->nrows >= ((int64_t) 1 << 31))
+            return dot3_sliced(C, M, mask_struct, A, B, sr, params, slice_max);
+    }
+    return dot3_dispatch(C, M, mask_struct, A, B, sr, params);
 }
 
 // sums over the mask entries (i,j) of min(|A(i,:)|, |B(j,:)|) and of

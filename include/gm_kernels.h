@@ -239,6 +239,11 @@ int64_t gm_arena_bytes_cached(void);   // freed blocks the arena holds for reuse
 // fails; n < 0 disables.  Mirrors the malloc_debug count-down of
 // Source/global/GB_Global.c:818-850.
 void gm_debug_fail_after(int64_t n);
+// gm_dot3 runs a mask of 2^32 or more entries as row slices below that and
+// joins their results; tests lower the slice size to v entries (v <= 0
+// restores 2^32).  No counterpart in the reference (its CPU kernels index
+// with 64-bit integers throughout).
+void gm_debug_dot3_slice_max(int64_t v);
 
 // The stream this thread's library calls run on.  gm_set_stream(s) makes
 // every later call of this thread enqueue its work on the caller's stream s

@@ -278,3 +278,34 @@ def test_dot3_work_model(grb):
                                  ctypes.byref(dA.view), ctypes.byref(dB.view))
     assert rc == 0, capi.last_error()
     assert mn.value == int(np.minimum(a, b).sum()) and ln.value == int((a + b).sum())
+
+
+@pytest.mark.parametrize("slice_max", [1, 700, 5000])
+def test_dot3_row_slices(grb, slice_max):
+    """Masks beyond the 2^32 entries one run packs are cut into row slices and
+    the slices' results joined; the slice size is lowered here so that small
+    masks take that path: a valued mask, an iso result and the triangle
+    count through GrB_mxm, all against the oracle."""
+    print(f"[dot3] row slices of {slice_max} entries", flush=True)
+    L = capi.lib()
+    rng = np.random.default_rng(9)
+    A = rand_csr(300, 150, 0.06, rng, "int32")
+    B = rand_csr(250, 150, 0.06, rng, "int32")
+    M = rand_csr(300, 250, 0.08, rng, "int64")
+    M.x[rng.random(M.nnz) < 0.3] = 0
+    _, Lr, Ur = W.tricount_inputs(9, 16, seed=2)
+    L.gm_debug_dot3_slice_max(slice_max)
+    try:
+        ref = O.dot3("min_plus_int32", M, A, B, mask_struct=False)
+        got = gm_dot3("GrB_MIN_PLUS_SEMIRING_INT32", M, A, B, mask_struct=False)
+        assert_csr_equal(got, ref, what="sliced valued mask")
+        ref = O.dot3("any_pair_bool", M, A, B)
+        got = gm_dot3("GxB_ANY_PAIR_BOOL", M, A, B)
+        assert_csr_equal(got, ref, what="sliced iso")
+        Ld, Ud = capi.Matrix.from_csr(Lr, "bool"), capi.Matrix.from_csr(Ur, "bool")
+        C = capi.Matrix("int64", Lr.nrows, Lr.nrows)
+        capi.mxm_checked(C, Ld, "GxB_PLUS_PAIR_INT64", Ld, Ud, "ST1")
+        ref = O.dot3("plus_pair_int64", Lr, Lr, Ur)
+        assert_csr_equal(C.to_csr(), ref, what="sliced tricount")
+    finally:
+        L.gm_debug_dot3_slice_max(0)
