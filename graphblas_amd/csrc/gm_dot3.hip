@@ -2463,13 +2463,13 @@ static int dot3_grouped_teams(Dot3Args &g, const gm_csr *A, const gm_csr *B)
 
 template <class S>
 static int dot3_run(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
-                    const gm_csr *B, const gm_dot3_params *prm)
+                    const gm_csr *B, const gm_dot3_params *prm, int64_t mnz_known = -1)
 {
     using Z = typename S::Z;
     hipStream_t st = stream();
-    const int64_t n = M->nrows, mnz = (n > 0) ? -1 : 0;
-    (void) mnz;
-    int64_t Mnz = 0;
+    const int64_t n = M->nrows;
+    int64_t Mnz = (n > 0) ? mnz_known : 0;
+    if (Mnz < 0)
     {
         // nnz(M) from its last row pointer
         int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
@@ -2843,13 +2843,14 @@ extern "C" int gm_AxB_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, con
 }
 
 static int dot3_dispatch(gm_csr_out *C, const gm_csr *M, bool mask_struct, const gm_csr *A,
-                         const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params)
+                         const gm_csr *B, const gm_semiring *sr, const gm_dot3_params *params,
+                         int64_t mnz_known = -1)
 {
     int rc = GM_NO_VALUE;
 #define GM_DOT3_CASE(ADD, MULT, ZC, Z)                                         \
     if (sr->add_op == ADD && sr->mult_op == MULT && sr->ztype == ZC)           \
     {                                                                          \
-        rc = dot3_run<SR<ADD, MULT, Z>>(C, M, mask_struct, A, B, params);      \
+        rc = dot3_run<SR<ADD, MULT, Z>>(C, M, mask_struct, A, B, params, mnz_known); \
         if (rc == GM_SUCCESS) C->type = ZC;                                    \
         return rc;                                                             \
     }
@@ -2998,6 +2999,7 @@ extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const g
     if (!gm_dot3_branch(M, mask_struct, A, B, sr))
         return set_error(GM_NO_VALUE, "gm_dot3: case not handled by the GPU kernels");
     GM_TRY(ensure_init());
+    int64_t mnz = -1;
     if (M->nrows > 0 && M->p)
     {
         int64_t *hp = (int64_t *) pinned_scratch(sizeof(int64_t));
@@ -3005,11 +3007,12 @@ extern "C" int gm_dot3(gm_csr_out *C, const gm_csr *M, bool mask_struct, const g
         hipStream_t st = stream();
         GM_HIP(hipMemcpyAsync(hp, M->p + M->nrows, sizeof(int64_t), hipMemcpyDeviceToHost, st), "nnz(M)");
         GM_HIP(hipStreamSynchronize(st), "nnz(M) sync");
+        mnz = *hp;
         const int64_t slice_max = g_dot3_slice_max;
-        if (*hp >= slice_max || M->nrows >= ((int64_t) 1 << 31))
+        if (mnz >= slice_max || M->nrows >= ((int64_t) 1 << 31))
             return dot3_sliced(C, M, mask_struct, A, B, sr, params, slice_max);
     }
-    return dot3_dispatch(C, M, mask_struct, A, B, sr, params);
+    return dot3_dispatch(C, M, mask_struct, A, B, sr, params, mnz);
 }
 
 // sums over the mask entries (i,j) of min(|A(i,:)|, |B(j,:)|) and of
