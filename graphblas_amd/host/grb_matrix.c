@@ -1064,7 +1064,9 @@ static GrB_Info GB_blob_parse(const uint8_t *blob, size_t size, GB_blob_csr *c)
     const int sparsity = sic / 4;
     const bool iso = (sic & 2) != 0, csc = (sic & 1) != 0;
     const int64_t lim = (int64_t) 1 << 60;
-    if (vlen < 0 || vdim < 0 || vlen >= lim || vdim >= lim || nvals < 0 || nvec < 0 || nvec > vdim)
+    // every size below times an element size stays under 2^63
+    if (vlen < 0 || vdim < 0 || vlen >= lim || vdim >= lim || nvals < 0 || nvals >= lim || nvec < 0 ||
+        nvec > vdim)
         return GrB_INVALID_OBJECT;
     int64_t nball = 0;
     for (int a = 0; a < 5; a++)
@@ -1127,7 +1129,7 @@ static GrB_Info GB_blob_parse(const uint8_t *blob, size_t size, GB_blob_csr *c)
         // every position of a bitmap that holds an entry, or of a full matrix
         const double cells = (double) vlen * (double) vdim;
         const int64_t nc = (int64_t) cells;
-        if (cells > (double) lim || p_len != 0 || h_len != 0 || i_len != 0 ||
+        if (cells >= (double) lim || p_len != 0 || h_len != 0 || i_len != 0 ||
             (sparsity == GxB_BITMAP && b_len != nc) || (sparsity == GxB_FULL && b_len != 0) ||
             x_len != typesize * (iso ? 1 : nc) || (sparsity == GxB_FULL && nvals != nc))
             info = GrB_INVALID_OBJECT;

@@ -314,3 +314,9 @@ def test_deserialize_rejects_bad_blobs():
     # a back-reference before the start of the output
     assert des(BL.blob(3, 2, 8, 8, arrs, nvals=1, methods={"i": BL.LZ4},
                        blocks={"i": [bytes([0x14, 1, 9, 0])]})) == capi.GrB_INVALID_OBJECT
+    # an entry count whose byte sizes wrap past 2^64 to the arrays' real
+    # 8 bytes (row pointers ending at that count): rejected before any entry
+    # is read
+    big = (1 << 61) + 1
+    assert des(BL.blob(3, 1, 8, 8, BL.csr_arrays([0, big], [2], [7]), nvals=big)) == \
+        capi.GrB_INVALID_OBJECT
