@@ -211,6 +211,15 @@ def lib():
         "GrB_Matrix_build_BOOL": ([P, P, P, P, U64, P], ctypes.c_int),
         "GrB_Matrix_extractTuples_INT64": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
         "GrB_Matrix_extractTuples_FP64": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_extractTuples_INT32": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_extractTuples_BOOL": ([P, P, P, ctypes.POINTER(U64), P], ctypes.c_int),
+        "GrB_Matrix_reduce_BOOL": ([ctypes.POINTER(ctypes.c_bool), P, P, P, P], ctypes.c_int),
+        "GxB_Matrix_type": ([PP, P], ctypes.c_int),
+        "GxB_Type_size": ([ctypes.POINTER(ctypes.c_size_t), P], ctypes.c_int),
+        "GrB_Type_get_size": ([ctypes.POINTER(ctypes.c_size_t), P], ctypes.c_int),
+        "GxB_Global_Option_get_INT32": ([ctypes.c_int, ctypes.POINTER(I32)], ctypes.c_int),
+        "GrB_BinaryOp_free": ([PP], ctypes.c_int),
+        "GrB_Monoid_free": ([PP], ctypes.c_int),
         "GB_mxm_select": ([ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool,
                            ctypes.c_int], ctypes.c_int),
         "GB_semiring_codes": ([P, ctypes.POINTER(GmSemiring)], ctypes.c_bool),
@@ -306,6 +315,10 @@ def lib():
         "GrB_Vector_extractTuples_FP64": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
         "GrB_Vector_extractTuples_BOOL": ([P, P, ctypes.POINTER(U64), P], ctypes.c_int),
         "GrB_Vector_reduce_INT64": ([ctypes.POINTER(I64), P, P, P, P], ctypes.c_int),
+        "GrB_Vector_reduce_INT32": ([ctypes.POINTER(I32), P, P, P, P], ctypes.c_int),
+        "GrB_Vector_reduce_FP64": ([ctypes.POINTER(ctypes.c_double), P, P, P, P], ctypes.c_int),
+        "GrB_Vector_reduce_BOOL": ([ctypes.POINTER(ctypes.c_bool), P, P, P, P], ctypes.c_int),
+        "GrB_Vector_dup": ([PP, P], ctypes.c_int),
         "GrB_Vector_error": ([ctypes.POINTER(ctypes.c_char_p), P], ctypes.c_int),
         "GrB_vxm": ([P, P, P, P, P, P, P], ctypes.c_int),
         "GrB_mxv": ([P, P, P, P, P, P, P], ctypes.c_int),

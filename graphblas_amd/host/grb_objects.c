@@ -166,9 +166,13 @@ GB_OPDEF(GxB_FIRSTJ_INT32, GB_OP_FIRSTJ, INT32, INT32, INT32)
 GB_OPDEF(GxB_SECONDI_INT32, GB_OP_SECONDI, INT32, INT32, INT32)
 GB_OPDEF(GxB_SECONDJ_INT32, GB_OP_SECONDJ, INT32, INT32, INT32)
 
+// Every operator here is a static built-in: freeing one changes nothing, and
+// the handle is left as it is, as the reference leaves a built-in's handle
+// (Source/op/GB_Op_free.c frees and clears only allocated operators), so a
+// caller that frees the address of a global handle keeps a working global.
 GrB_Info GrB_BinaryOp_free(GrB_BinaryOp *op)
 {
-    if (op) *op = NULL;      // built-ins are static; nothing else can be made
+    (void) op;
     return GrB_SUCCESS;
 }
 
@@ -196,9 +200,10 @@ GB_MONDEF(GxB_ANY_BOOL_MONOID, GxB_ANY_BOOL)
 GB_MONDEF(GrB_LOR_MONOID_BOOL, GrB_LOR)
 GB_MONDEF(GrB_LAND_MONOID_BOOL, GrB_LAND)
 
+// built-in monoids only, left as they are (as GrB_BinaryOp_free)
 GrB_Info GrB_Monoid_free(GrB_Monoid *monoid)
 {
-    if (monoid) *monoid = NULL;
+    (void) monoid;
     return GrB_SUCCESS;
 }
 
@@ -247,12 +252,14 @@ GrB_Info GrB_Semiring_free(GrB_Semiring *semiring)
 {
     if (!semiring) return GrB_SUCCESS;
     GrB_Semiring s = *semiring;
+    // a built-in keeps its handle (the reference frees and clears only
+    // allocated semirings, Source/semiring/GrB_Semiring_free.c)
     if (s && s->magic == GB_MAGIC && !s->builtin)
     {
         s->magic = GB_FREED;
         free(s);
+        *semiring = NULL;
     }
-    *semiring = NULL;
     return GrB_SUCCESS;
 }
 
@@ -391,11 +398,12 @@ GrB_Info GrB_Descriptor_free(GrB_Descriptor *descriptor)
 {
     if (!descriptor) return GrB_SUCCESS;
     GrB_Descriptor d = *descriptor;
+    // a built-in (GrB_DESC_*) keeps its handle, as in the reference
     if (d && d->magic == GB_MAGIC && !d->builtin)
     {
         d->magic = GB_FREED;
         free(d);
+        *descriptor = NULL;
     }
-    *descriptor = NULL;
     return GrB_SUCCESS;
 }

@@ -201,6 +201,58 @@ def test_semiring_new_domain_check():
     c = capi.GmSemiring()
     assert L.GB_semiring_codes(s, ctypes.byref(c)) and (c.add_op, c.mult_op, c.ztype) == (2, 2, 8)
     assert L.GrB_Semiring_free(ctypes.byref(s)) == 0
+    assert s.value is None
+
+
+def test_freeing_builtins_keeps_them():
+    """Freeing a built-in operator, monoid, semiring or descriptor changes
+    nothing and leaves the handle as it was (the reference frees and clears
+    only allocated objects, Source/op/GB_Op_free.c,
+    Source/semiring/GrB_Semiring_free.c), even when the handle passed is the
+    library's own global."""
+    L = capi.lib()
+    capi.init()
+    for fn, name in (("GrB_BinaryOp_free", "GrB_PLUS_INT64"),
+                     ("GrB_Monoid_free", "GrB_PLUS_MONOID_INT64"),
+                     ("GrB_Semiring_free", "GxB_PLUS_PAIR_INT64"),
+                     ("GrB_Descriptor_free", "GrB_DESC_ST1")):
+        before = capi.handle(name)
+        copy = ctypes.c_void_p(before)
+        assert getattr(L, fn)(ctypes.byref(copy)) == 0
+        assert copy.value == before, fn
+        glob = ctypes.c_void_p.in_dll(L, name)
+        assert getattr(L, fn)(ctypes.byref(glob)) == 0
+        assert capi.handle(name) == before, fn
+    # the semiring and descriptor still work after their globals were freed
+    c = capi.GmSemiring()
+    assert L.GB_semiring_codes(ctypes.c_void_p(capi.handle("GxB_PLUS_PAIR_INT64")), ctypes.byref(c))
+    assert L.GrB_Descriptor_set(capi.desc("ST1"), 0, 1) == capi.GrB_INVALID_VALUE
+
+
+def test_type_sizes_and_options():
+    L = capi.lib()
+    capi.init()
+    for name, size in (("GrB_BOOL", 1), ("GrB_INT8", 1), ("GrB_UINT16", 2), ("GrB_INT32", 4),
+                       ("GrB_UINT32", 4), ("GrB_INT64", 8), ("GrB_FP32", 4), ("GrB_FP64", 8)):
+        s1, s2 = ctypes.c_size_t(), ctypes.c_size_t()
+        t = ctypes.c_void_p(capi.handle(name))
+        assert L.GxB_Type_size(ctypes.byref(s1), t) == 0 and s1.value == size, name
+        assert L.GrB_Type_get_size(ctypes.byref(s2), t) == 0 and s2.value == size, name
+    assert L.GxB_Type_size(None, ctypes.c_void_p(capi.handle("GrB_FP64"))) == capi.GrB_NULL_POINTER
+    A = _mat("int32", 3, 4)
+    t = ctypes.c_void_p()
+    assert L.GxB_Matrix_type(ctypes.byref(t), A.h) == 0
+    assert t.value == capi.handle("GrB_INT32")
+    v = ctypes.c_int32()
+    GxB_BURBLE = 7019
+    assert L.GxB_Global_Option_get_INT32(GxB_BURBLE, ctypes.byref(v)) == 0
+    old = v.value
+    try:
+        assert L.GxB_Global_Option_set_INT32(GxB_BURBLE, 1) == 0
+        assert L.GxB_Global_Option_get_INT32(GxB_BURBLE, ctypes.byref(v)) == 0 and v.value == 1
+    finally:
+        L.GxB_Global_Option_set_INT32(GxB_BURBLE, old)
+    assert L.GxB_Global_Option_get_INT32(12345, ctypes.byref(v)) == capi.GrB_INVALID_VALUE
 
 
 def test_generators_are_well_formed():
